@@ -1,0 +1,591 @@
+// capi.cpp -- the extern "C" entry points of libdtc_amd.so (include/dtc_capi.h).
+//
+// Host-side pieces here are the O(24^3) fit of calc_effective_material_parameters
+// (f-src/mod_struct_calcmat.f90:216-473): the load-case matrix vv and its
+// inverse, K_M = ff vv^-1, the face-averaged forces fv, C = fv meps and
+// EE = (C + C^T)/2.  Everything proportional to the mesh size runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bc.hpp"
+#include "dtc_internal.hpp"
+#include "io.hpp"
+#include "rotation.hpp"
+#include "solver.hpp"
+
+namespace dtc {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+namespace {
+
+constexpr int R = R24;
+
+struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() { if (s) (void)hipStreamDestroy(s); }
+    int create() {
+        hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e != hipSuccess) return set_error(DTC_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+        return DTC_OK;
+    }
+};
+
+int kind_size(int kind) { return kind == DTC_IK1 ? 1 : kind == DTC_IK2 ? 2 : kind == DTC_IK4 ? 4 : 0; }
+
+int check_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return set_error(DTC_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    return DTC_OK;
+}
+
+// ---- load-case tables (init_displ_hexe8 / init_loadcase_el_order_lin)
+// vv(dof, lc) with dof = d*8 + l (f-src/mod_struct_calcmat.f90:1684-1770); the
+// entries are the same (d, l, m) list as the boundary table.
+void vv_lin(double rve_strain, double vv[576]) {
+    std::memset(vv, 0, sizeof(double) * 576);
+    for (int lc = 0; lc < R; ++lc) {
+        int cnt;
+        const UUEntry* e = uu_entries(lc, &cnt);
+        for (int k = 0; k < cnt; ++k) vv[(e[k].d * 8 + e[k].l) + R * lc] = (double)e[k].m * rve_strain;
+    }
+}
+
+int invert(double* A, int n) {
+    std::vector<double> M(A, A + n * n), I(n * n, 0.0);
+    for (int i = 0; i < n; ++i) I[i + n * i] = 1.0;
+    for (int k = 0; k < n; ++k) {
+        int pr = k;
+        double pv = std::fabs(M[k + n * k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(M[i + n * k]) > pv) { pv = std::fabs(M[i + n * k]); pr = i; }
+        if (pv == 0.0) return set_error(DTC_ERR_BAD_ARGUMENT, "singular load-case matrix");
+        if (pr != k)
+            for (int j = 0; j < n; ++j) { std::swap(M[k + n * j], M[pr + n * j]); std::swap(I[k + n * j], I[pr + n * j]); }
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            double f = M[i + n * k] / M[k + n * k];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; ++j) { M[i + n * j] -= f * M[k + n * j]; I[i + n * j] -= f * I[k + n * j]; }
+        }
+    }
+    for (int k = 0; k < n; ++k) {
+        double d = M[k + n * k];
+        for (int j = 0; j < n; ++j) I[k + n * j] /= d;
+    }
+    std::memcpy(A, I.data(), sizeof(double) * n * n);
+    return DTC_OK;
+}
+
+// fv = W ff(:, 1:6), HEX08 macro (f-src/mod_struct_calcmat.f90:345-359)
+void fv_weights(const double xdp[3], double W[6 * 24]) {
+    std::memset(W, 0, sizeof(double) * 6 * 24);
+    const double A23 = xdp[1] * xdp[2], A13 = xdp[0] * xdp[2], A12 = xdp[0] * xdp[1];
+    auto set = [&](int i, std::initializer_list<int> rows, double w) { for (int r : rows) W[i + 6 * (r - 1)] += w; };
+    set(0, {2, 3, 6, 7}, 1.0 / A23);
+    set(1, {11, 12, 15, 16}, 1.0 / A13);
+    set(2, {21, 22, 23, 24}, 1.0 / A12);
+    set(3, {3, 4, 7, 8}, 0.5 / A13);
+    set(3, {10, 11, 14, 15}, 0.5 / A23);
+    set(4, {5, 6, 7, 8}, 0.5 / A12);
+    set(4, {18, 19, 22, 23}, 0.5 / A23);
+    set(5, {13, 14, 15, 16}, 0.5 / A12);
+    set(5, {19, 20, 23, 24}, 0.5 / A13);
+}
+
+constexpr double MEPS[6] = {1.e6, 1.e6, 1.e6, -1.e6, -1.e6, 1.e6};   // :434-441
+
+int effective(const double* ff, const double xdp[3], double rve_strain, double* K_M, double* C, double* EE) {
+    double vv[576];
+    vv_lin(rve_strain, vv);
+    DTC_TRY(invert(vv, R));
+    for (int i = 0; i < R; ++i)
+        for (int j = 0; j < R; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < R; ++k) s += ff[i + R * k] * vv[k + R * j];
+            K_M[i + R * j] = s;
+        }
+    double W[144], fv[36];
+    fv_weights(xdp, W);
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+            for (int a = 0; a < R; ++a) s += W[i + 6 * a] * ff[a + R * j];
+            fv[i + 6 * j] = s;
+        }
+    for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) C[i + 6 * j] = fv[i + 6 * j] * MEPS[j];
+    for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) EE[i + 6 * j] = 0.5 * (C[i + 6 * j] + C[j + 6 * i]);
+    return DTC_OK;
+}
+
+// HEX20 element stiffness (3x3x3 Gauss) in gen_quadmesh node order
+int element_stiffness20(double E, double nu, double a, double* K) {
+    static const int OFF[20][3] = {
+        {0, 0, 0}, {1, 0, 0}, {2, 0, 0}, {2, 1, 0}, {2, 2, 0}, {1, 2, 0}, {0, 2, 0}, {0, 1, 0},
+        {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1},
+        {0, 0, 2}, {1, 0, 2}, {2, 0, 2}, {2, 1, 2}, {2, 2, 2}, {1, 2, 2}, {0, 2, 2}, {0, 1, 2}};
+    const int nd = 60;
+    double D[6][6] = {};
+    const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) D[i][j] = lam;
+    for (int i = 0; i < 3; ++i) D[i][i] += 2.0 * mu;
+    D[3][3] = D[4][4] = D[5][5] = mu;
+    std::memset(K, 0, sizeof(double) * nd * nd);
+    const double gp[3] = {-std::sqrt(0.6), 0.0, std::sqrt(0.6)}, gw[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    const double jac = 2.0 / a, detJ = (a / 2.0) * (a / 2.0) * (a / 2.0);
+    static thread_local double B[6][60];
+    for (int qi = 0; qi < 27; ++qi) {
+        const double q[3] = {gp[qi % 3], gp[(qi / 3) % 3], gp[qi / 9]};
+        const double w = gw[qi % 3] * gw[(qi / 3) % 3] * gw[qi / 9] * detJ;
+        std::memset(B, 0, sizeof(B));
+        for (int l = 0; l < 20; ++l) {
+            double s[3] = {OFF[l][0] - 1.0, OFF[l][1] - 1.0, OFF[l][2] - 1.0};
+            int zero = -1;
+            for (int d = 0; d < 3; ++d) if (s[d] == 0.0) zero = d;
+            double dN[3];
+            if (zero < 0) {
+                double f0 = 1 + s[0] * q[0], f1 = 1 + s[1] * q[1], f2 = 1 + s[2] * q[2];
+                double t = s[0] * q[0] + s[1] * q[1] + s[2] * q[2] - 2.0;
+                dN[0] = (s[0] * f1 * f2 * t + f0 * f1 * f2 * s[0]) / 8.0;
+                dN[1] = (f0 * s[1] * f2 * t + f0 * f1 * f2 * s[1]) / 8.0;
+                dN[2] = (f0 * f1 * s[2] * t + f0 * f1 * f2 * s[2]) / 8.0;
+            } else {
+                double f[3], df[3];
+                for (int d = 0; d < 3; ++d) {
+                    if (d == zero) { f[d] = 1 - q[d] * q[d]; df[d] = -2 * q[d]; }
+                    else { f[d] = 1 + s[d] * q[d]; df[d] = s[d]; }
+                }
+                dN[0] = df[0] * f[1] * f[2] / 4.0;
+                dN[1] = f[0] * df[1] * f[2] / 4.0;
+                dN[2] = f[0] * f[1] * df[2] / 4.0;
+            }
+            const double dx = dN[0] * jac, dy = dN[1] * jac, dz = dN[2] * jac;
+            B[0][3 * l] = dx; B[1][3 * l + 1] = dy; B[2][3 * l + 2] = dz;
+            B[3][3 * l] = dy; B[3][3 * l + 1] = dx;
+            B[4][3 * l] = dz; B[4][3 * l + 2] = dx;
+            B[5][3 * l + 1] = dz; B[5][3 * l + 2] = dy;
+        }
+        for (int r = 0; r < nd; ++r)
+            for (int m = 0; m < 6; ++m) {
+                if (B[m][r] == 0.0) continue;
+                for (int p = 0; p < 6; ++p) {
+                    const double f = w * B[m][r] * D[m][p];
+                    if (f == 0.0) continue;
+                    for (int c = 0; c < nd; ++c) K[r + nd * c] += f * B[p][c];
+                }
+            }
+    }
+    return DTC_OK;
+}
+
+int validate_desc(const dtc_domain_desc& d) {
+    if (!d.phi) return set_error(DTC_ERR_BAD_ARGUMENT, "domain voxels are NULL");
+    if (!kind_size(d.phi_kind)) return set_error(DTC_ERR_BAD_ARGUMENT, "voxel kind %d (expected 1, 2 or 4)", d.phi_kind);
+    for (int k = 0; k < 3; ++k) {
+        if (d.n[k] <= 0 || d.n[k] > 4096) return set_error(DTC_ERR_BAD_ARGUMENT, "domain extent n[%d] = %lld", k, (long long)d.n[k]);
+        if (!(d.delta[k] > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "grid spacing delta[%d] = %g", k, d.delta[k]);
+    }
+    if (d.micro_nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "the solve implements HEX08 micro elements (got %d nodes)", d.micro_nen);
+    if (d.macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "the solve implements the HEX08 macro element (order %d)", d.macro_order);
+    if (!(d.nu > -1.0 && d.nu < 0.5) || !(d.E > 0.0) || !(d.size_domain > 0.0))
+        return set_error(DTC_ERR_BAD_ARGUMENT, "material E=%g nu=%g size_domain=%g", d.E, d.nu, d.size_domain);
+    return DTC_OK;
+}
+
+}  // namespace
+}  // namespace dtc
+
+using namespace dtc;
+
+// ---------------------------------------------------------------- batch object
+struct dtc_batch {
+    std::vector<dtc_domain_desc> d;
+    dtc_solve_opts o{};
+    Stream st;
+    std::vector<std::unique_ptr<DevMem>> phi, mod;
+    std::vector<double> escale;
+    SolverBatch sb;
+    MeshDev mesh;
+    std::vector<double> ff, EE;
+    std::vector<int64_t> iters, outer;
+    std::vector<double> relres;
+    double run_ms = 0.0;
+    bool ran = false;
+};
+
+extern "C" {
+
+const char* dtc_last_error(void) { return g_err.c_str(); }
+int dtc_version(void) { return 10000; }
+
+int dtc_ddc_params(const double size_domain[3], const double delta[3], const int64_t vdim[3], int64_t x_D[3],
+                   int64_t nn_D[3], double x_D_phy[3]) {
+    if (!size_domain || !delta || !vdim || !x_D || !nn_D || !x_D_phy) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL argument");
+    int64_t xd[3], nd[3];
+    double xp[3];
+    for (int k = 0; k < 3; ++k) {
+        if (!(delta[k] > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "delta[%d] = %g", k, delta[k]);
+        xd[k] = (int64_t)std::llround(size_domain[k] / delta[k]);   // NINT
+        if (xd[k] <= 0) return set_error(DTC_ERR_BAD_ARGUMENT, "SIZE_DOMAIN/SPACING rounds to %lld", (long long)xd[k]);
+        xp[k] = (double)xd[k] * delta[k];
+        nd[k] = (vdim[k] - 2) / xd[k];                              // bpoints = 1
+    }
+    for (int k = 0; k < 3; ++k) { x_D[k] = xd[k]; nn_D[k] = nd[k]; x_D_phy[k] = xp[k]; }
+    return DTC_OK;
+}
+
+int dtc_domain_bounds(int64_t domain, const int64_t x_D[3], const int64_t nn_D[3], int64_t nn_i[3], int64_t xa_n[3],
+                      int64_t xe_n[3]) {
+    if (!x_D || !nn_D || !nn_i || !xa_n || !xe_n) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL argument");
+    if (domain < 0 || nn_D[0] <= 0 || nn_D[1] <= 0 || nn_D[2] <= 0 || domain >= nn_D[0] * nn_D[1] * nn_D[2])
+        return set_error(DTC_ERR_BAD_ARGUMENT, "domain %lld outside the decomposition", (long long)domain);
+    const int64_t n3 = domain / (nn_D[0] * nn_D[1]);
+    const int64_t n2 = (domain - nn_D[0] * nn_D[1] * n3) / nn_D[0];
+    const int64_t n1 = domain - nn_D[0] * nn_D[1] * n3 - nn_D[0] * n2;
+    const int64_t nn[3] = {n1, n2, n3};
+    for (int k = 0; k < 3; ++k) {
+        nn_i[k] = nn[k];
+        xa_n[k] = x_D[k] * nn[k] + 2;
+        xe_n[k] = x_D[k] * (nn[k] + 1) + 1;
+    }
+    return DTC_OK;
+}
+
+int dtc_extract_block(const void* vol, int kind, const int64_t vdim[3], const int64_t xa_n[3], const int64_t xe_n[3],
+                      void* out) {
+    if (!vol || !vdim || !xa_n || !xe_n || !out || !kind_size(kind)) return set_error(DTC_ERR_BAD_ARGUMENT, "bad argument");
+    DTC_TRY(check_device());
+    const int es = kind_size(kind);
+    int64_t n[3];
+    for (int k = 0; k < 3; ++k) {
+        n[k] = xe_n[k] - xa_n[k] + 1;
+        if (n[k] <= 0 || xa_n[k] < 1 || xe_n[k] > vdim[k]) return set_error(DTC_ERR_BAD_ARGUMENT, "block outside the volume");
+    }
+    Stream st;
+    DTC_TRY(st.create());
+    DevMem dv, dout;
+    const size_t vb = (size_t)vdim[0] * vdim[1] * vdim[2] * es, ob = (size_t)n[0] * n[1] * n[2] * es;
+    DTC_TRY(dv.alloc(vb));
+    DTC_TRY(dout.alloc(ob));
+    DTC_HIP(hipMemcpyAsync(dv.p, vol, vb, hipMemcpyHostToDevice, st.s));
+    DTC_TRY(extract_block_device(dv.p, kind, vdim, xa_n, xe_n, dout.p, st.s));
+    DTC_HIP(hipMemcpyAsync(out, dout.p, ob, hipMemcpyDeviceToHost, st.s));
+    DTC_HIP(hipStreamSynchronize(st.s));
+    return DTC_OK;
+}
+
+static int mesh_common(const void* phi, int kind, const int64_t n[3], int64_t llimit, int nen, MeshDev& m, Stream& st,
+                       DevMem& dphi) {
+    if (!phi || !n || !kind_size(kind)) return set_error(DTC_ERR_BAD_ARGUMENT, "bad argument");
+    for (int k = 0; k < 3; ++k)
+        if (n[k] <= 0 || n[k] > 4096) return set_error(DTC_ERR_BAD_ARGUMENT, "n[%d] = %lld", k, (long long)n[k]);
+    DTC_TRY(check_device());
+    DTC_TRY(st.create());
+    const size_t bytes = (size_t)n[0] * n[1] * n[2] * kind_size(kind);
+    DTC_TRY(dphi.alloc(bytes));
+    DTC_HIP(hipMemcpyAsync(dphi.p, phi, bytes, hipMemcpyHostToDevice, st.s));
+    return mesh_build_device(dphi.p, kind, n, llimit, nen, m, st.s);
+}
+
+int dtc_mesh_count(const void* phi, int kind, const int64_t n[3], int64_t llimit, int nen, int64_t* n_elems,
+                   int64_t* n_nodes) {
+    if (!n_elems || !n_nodes) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL output");
+    MeshDev m;
+    Stream st;
+    DevMem dphi;
+    int rc = mesh_common(phi, kind, n, llimit, nen, m, st, dphi);
+    if (rc == DTC_ERR_EMPTY_MESH) { *n_elems = 0; *n_nodes = 0; return rc; }
+    DTC_TRY(rc);
+    *n_elems = m.n_elems;
+    *n_nodes = m.n_nodes;
+    return DTC_OK;
+}
+
+int dtc_mesh_build(const void* phi, int kind, const int64_t n[3], int64_t llimit, int nen, const double delta[3],
+                   const int64_t xa_n[3], int64_t elems_cap, int64_t nodes_cap, double* nodes, int64_t* elems,
+                   int64_t* n_elems, int64_t* n_nodes) {
+    if (!delta || !xa_n || !nodes || !elems || !n_elems || !n_nodes) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL argument");
+    MeshDev m;
+    Stream st;
+    DevMem dphi;
+    DTC_TRY(mesh_common(phi, kind, n, llimit, nen, m, st, dphi));
+    if (m.n_elems > elems_cap || m.n_nodes > nodes_cap)
+        return set_error(DTC_ERR_BAD_ARGUMENT, "mesh has %lld elements / %lld nodes, capacity %lld / %lld",
+                         (long long)m.n_elems, (long long)m.n_nodes, (long long)elems_cap, (long long)nodes_cap);
+    DevMem dn, de;
+    DTC_TRY(dn.alloc((size_t)m.n_nodes * 3 * 8));
+    DTC_TRY(de.alloc((size_t)m.n_elems * nen * 8));
+    DTC_TRY(mesh_write_outputs(m, delta, xa_n, dn.as<double>(), de.as<int64_t>(), st.s));
+    DTC_HIP(hipMemcpyAsync(nodes, dn.p, (size_t)m.n_nodes * 3 * 8, hipMemcpyDeviceToHost, st.s));
+    DTC_HIP(hipMemcpyAsync(elems, de.p, (size_t)m.n_elems * nen * 8, hipMemcpyDeviceToHost, st.s));
+    DTC_HIP(hipStreamSynchronize(st.s));
+    *n_elems = m.n_elems;
+    *n_nodes = m.n_nodes;
+    return DTC_OK;
+}
+
+int dtc_boundaries(const double* nodes, int64_t n_nodes, const double min_c[3], const double max_c[3],
+                   const double dim_c[3], int macro_order, int64_t* n_b, int64_t* bnode_ids, double* bvals) {
+    if (!nodes || !min_c || !max_c || !dim_c || !n_b || n_nodes < 0) return set_error(DTC_ERR_BAD_ARGUMENT, "bad argument");
+    if (macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "boundary tables implement the HEX08 macro element");
+    if ((bnode_ids == nullptr) != (bvals == nullptr)) return set_error(DTC_ERR_BAD_ARGUMENT, "ids and values go together");
+    DTC_TRY(check_device());
+    Stream st;
+    DTC_TRY(st.create());
+    DevMem dn, flag, pos, tmp, di, dv;
+    DTC_TRY(dn.alloc((size_t)n_nodes * 3 * 8));
+    DTC_HIP(hipMemcpyAsync(dn.p, nodes, (size_t)n_nodes * 3 * 8, hipMemcpyHostToDevice, st.s));
+    int64_t nb = 0;
+    DTC_TRY(boundaries_device(dn.as<double>(), n_nodes, min_c, max_c, dim_c, &nb, nullptr, nullptr, flag, pos, tmp, st.s));
+    if (bnode_ids) {
+        DTC_TRY(di.alloc((size_t)nb * 8));
+        DTC_TRY(dv.alloc((size_t)nb * 3 * R * 8));
+        DTC_TRY(boundaries_device(dn.as<double>(), n_nodes, min_c, max_c, dim_c, &nb, di.as<int64_t>(), dv.as<double>(), flag,
+                                  pos, tmp, st.s));
+        DTC_HIP(hipMemcpyAsync(bnode_ids, di.p, (size_t)nb * 8, hipMemcpyDeviceToHost, st.s));
+        DTC_HIP(hipMemcpyAsync(bvals, dv.p, (size_t)nb * 3 * R * 8, hipMemcpyDeviceToHost, st.s));
+        DTC_HIP(hipStreamSynchronize(st.s));
+    }
+    *n_b = nb;
+    return DTC_OK;
+}
+
+int dtc_element_stiffness(int nen, double E, double nu, double a, double* K) {
+    if (!K) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL output");
+    if (!(a > 0.0) || !(nu > -1.0 && nu < 0.5)) return set_error(DTC_ERR_BAD_ARGUMENT, "a=%g nu=%g", a, nu);
+    if (nen == 8) return element_stiffness(8, E, nu, a, K);
+    if (nen == 20) return element_stiffness20(E, nu, a, K);
+    return set_error(DTC_ERR_BAD_ARGUMENT, "nen = %d", nen);
+}
+
+int dtc_batch_create(const dtc_domain_desc* d, int64_t n_dom, const dtc_solve_opts* o, dtc_batch** out) {
+    if (!d || !o || !out || n_dom <= 0) return set_error(DTC_ERR_BAD_ARGUMENT, "bad argument");
+    *out = nullptr;
+    for (int64_t i = 0; i < n_dom; ++i) DTC_TRY(validate_desc(d[i]));
+    for (int64_t i = 1; i < n_dom; ++i)
+        if (d[i].nu != d[0].nu) return set_error(DTC_ERR_UNSUPPORTED, "one Poisson ratio per batch");
+    if (o->precision != DTC_FP64 && o->precision != DTC_FP32_REFINE) return set_error(DTC_ERR_BAD_ARGUMENT, "precision %d", o->precision);
+    if (!(o->rel_tol > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "rel_tol %g", o->rel_tol);
+    DTC_TRY(check_device());
+    std::unique_ptr<dtc_batch> b(new dtc_batch);
+    b->d.assign(d, d + n_dom);
+    b->o = *o;
+    if (!(b->o.inner_tol > 0.0 && b->o.inner_tol < 1.0)) b->o.inner_tol = 1e-4;
+    if (!(b->o.rot_step_deg > 0.0)) b->o.rot_step_deg = 1.0;
+    if (!(b->o.num_zero > 0.0)) b->o.num_zero = 1e-9;
+    DTC_TRY(b->st.create());
+    const double a0 = d[0].size_domain * d[0].delta[0];   // a = phdsize * delta (f-src/mod_linfe.f90:180)
+    for (int64_t i = 0; i < n_dom; ++i) {
+        const size_t nv = (size_t)d[i].n[0] * d[i].n[1] * d[i].n[2];
+        std::unique_ptr<DevMem> p(new DevMem), m(new DevMem);
+        DTC_TRY(p->alloc(nv * kind_size(d[i].phi_kind)));
+        DTC_HIP(hipMemcpyAsync(p->p, d[i].phi, nv * kind_size(d[i].phi_kind), hipMemcpyHostToDevice, b->st.s));
+        if (d[i].modulus) {
+            DTC_TRY(m->alloc(nv * 8));
+            DTC_HIP(hipMemcpyAsync(m->p, d[i].modulus, nv * 8, hipMemcpyHostToDevice, b->st.s));
+        }
+        b->phi.push_back(std::move(p));
+        b->mod.push_back(std::move(m));
+        b->escale.push_back(d[i].size_domain * d[i].delta[0] / a0);
+        b->d[i].phi = nullptr;       // the host copy is not used after this call
+        b->d[i].modulus = nullptr;
+    }
+    DTC_HIP(hipStreamSynchronize(b->st.s));
+    b->sb.stream = b->st.s;
+    b->sb.nu = d[0].nu;
+    b->sb.a = a0;
+    *out = b.release();
+    return DTC_OK;
+}
+
+int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
+    if (!b || !res) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL argument");
+    const int D = (int)b->d.size();
+    hipStream_t s = b->st.s;
+    DTC_HIP(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    b->sb.timing = true;
+    DTC_TRY(batch_begin(b->sb, b->d.data(), D));
+    for (int i = 0; i < D; ++i) {
+        const dtc_domain_desc& d = b->d[i];
+        int rc = mesh_build_device(b->phi[i]->p, d.phi_kind, d.n, d.llimit, d.micro_nen, b->mesh, s);
+        if (rc == DTC_ERR_EMPTY_MESH) {
+            DTC_TRY(batch_add_empty(b->sb, d, DTC_ERR_EMPTY_MESH));
+            continue;
+        }
+        DTC_TRY(rc);
+        DTC_TRY(batch_add_domain(b->sb, b->mesh, d, b->mod[i]->p ? b->mod[i]->as<double>() : nullptr, b->escale[i], s));
+    }
+    DTC_TRY(batch_finalize(b->sb, b->o.precision));
+    DTC_TRY(batch_solve(b->sb, b->o, b->iters, b->outer, b->relres));
+    DTC_TRY(batch_forces(b->sb, b->ff));
+    b->EE.assign((size_t)D * 36, 0.0);
+    std::vector<dtc_domain_result> out(D);
+    std::vector<int> solved;
+    for (int i = 0; i < D; ++i) {
+        dtc_domain_result& r = out[i];
+        std::memset(&r, 0, sizeof(r));
+        const dtc_domain_desc& d = b->d[i];
+        r.status = b->sb.h_status[i];
+        r.n_elems = b->sb.h_elems[i];
+        r.n_nodes = b->sb.h_nodes[i];
+        r.n_bnodes = b->sb.h_bnodes[i];
+        r.iterations = b->iters[i];
+        r.outer_iterations = b->outer[i];
+        r.relres_max = b->relres[i];
+        double xdp[3], v = 1.0;
+        for (int k = 0; k < 3; ++k) {
+            xdp[k] = (double)d.n[k] * d.delta[k];
+            v *= std::round(xdp[k] / d.delta[k]);
+        }
+        r.density = (double)r.n_elems / v;   // f-src/mod_struct_calcmat.f90:1642-1646
+        if (r.status != DTC_OK && r.status != DTC_ERR_NO_CONVERGENCE) continue;
+        std::memcpy(r.ff, &b->ff[(size_t)i * 576], sizeof(r.ff));
+        DTC_TRY(effective(r.ff, xdp, d.rve_strain, r.K_macro, r.C, r.EE));
+        std::memcpy(&b->EE[(size_t)i * 36], r.EE, sizeof(r.EE));
+        solved.push_back(i);
+    }
+    if (b->o.rotation && !solved.empty()) {
+        std::vector<double> EEs;
+        for (int i : solved) EEs.insert(EEs.end(), &b->EE[(size_t)i * 36], &b->EE[(size_t)i * 36] + 36);
+        std::vector<RotResult> rr;
+        DTC_TRY(rotation_search_batch(EEs.data(), (int)solved.size(), b->o.rot_step_deg, b->o.num_zero, s, rr));
+        for (size_t k = 0; k < solved.size(); ++k) {
+            dtc_domain_result& r = out[solved[k]];
+            for (int c = 0; c < 2; ++c) {
+                const RotResult& q = rr[2 * k + c];
+                r.rot_alpha[c] = q.alpha;
+                std::memcpy(&r.rot_n[3 * c], q.n, sizeof(q.n));
+                std::memcpy(&r.rot_aa[9 * c], q.aa, sizeof(q.aa));
+                std::memcpy(&r.rot_EE[36 * c], q.EE_opt, sizeof(q.EE_opt));
+                r.rot_jj[c] = q.jj;
+                r.rot_crit[c] = q.crit;
+            }
+        }
+    }
+    DTC_HIP(hipStreamSynchronize(s));
+    b->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::memcpy(res, out.data(), sizeof(dtc_domain_result) * D);
+    b->ran = true;
+    return DTC_OK;
+}
+
+int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_dE) {
+    if (!b || !dL_dC || !dL_dE) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL argument");
+    if (!b->ran) return set_error(DTC_ERR_BAD_ARGUMENT, "dtc_batch_grad before dtc_batch_run");
+    if (i < 0 || i >= (int64_t)b->d.size()) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %lld", (long long)i);
+    const dtc_domain_desc& d = b->d[i];
+    const size_t nv = (size_t)d.n[0] * d.n[1] * d.n[2];
+    if (b->sb.h_status[i] != DTC_OK && b->sb.h_status[i] != DTC_ERR_NO_CONVERGENCE) {
+        std::memset(dL_dE, 0, nv * 8);
+        return DTC_OK;
+    }
+    // M(b,j) = meps_j sum_i G_ij sum_a W_ia vv_eps^-1(b,a); vv at eps = 1e-6, the
+    // value the boundary tables use (f-src/mod_write_deck.f90:68)
+    double vv[576], W[144], M[144];
+    vv_lin(1.e-6, vv);
+    DTC_TRY(invert(vv, R));
+    double xdp[3] = {(double)d.n[0] * d.delta[0], (double)d.n[1] * d.delta[1], (double)d.n[2] * d.delta[2]};
+    fv_weights(xdp, W);
+    for (int bb = 0; bb < R; ++bb)
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+            for (int ii = 0; ii < 6; ++ii) {
+                double t = 0.0;
+                for (int a = 0; a < R; ++a) t += W[ii + 6 * a] * vv[bb + R * a];
+                s += dL_dC[ii + 6 * j] * t;
+            }
+            M[bb * 6 + j] = s * MEPS[j];
+        }
+    std::vector<double> g(nv);
+    DTC_TRY(batch_grad(b->sb, (int)i, M, g.data()));
+    const double sc = b->escale[i];   // d(emod)/d(E_v)
+    for (size_t v = 0; v < nv; ++v) dL_dE[v] = g[v] * sc;
+    return DTC_OK;
+}
+
+int dtc_batch_stats(dtc_batch* b, double* mv_avg_ms, int64_t* mv_launches, double* mv_bytes, double* mv_flops,
+                    double* run_ms) {
+    if (!b) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL batch");
+    const double avg = b->sb.mv_count ? b->sb.mv_ms_sum / (double)b->sb.mv_count : 0.0;
+    int64_t ne = 0, nn = 0;
+    for (size_t i = 0; i < b->sb.h_elems.size(); ++i)
+        if (b->sb.h_status[i] == DTC_OK || b->sb.h_status[i] == DTC_ERR_NO_CONVERGENCE) {
+            ne += b->sb.h_elems[i];
+            nn += b->sb.h_nodes[i];
+        }
+    const double s = b->o.precision == DTC_FP64 ? 8.0 : 4.0;
+    if (mv_avg_ms) *mv_avg_ms = avg;
+    if (mv_launches) *mv_launches = b->sb.mv_count;
+    // B_mv = 2 (3 N_n R s) + 4 N_e, F_mv = 1152 R N_e (dense-equivalent)
+    if (mv_bytes) *mv_bytes = 2.0 * (3.0 * (double)nn * R * s) + 4.0 * (double)ne;
+    if (mv_flops) *mv_flops = 1152.0 * R * (double)ne;
+    if (run_ms) *run_ms = b->run_ms;
+    return DTC_OK;
+}
+
+int dtc_batch_destroy(dtc_batch* b) {
+    if (!b) return DTC_OK;
+    if (b->st.s) (void)hipStreamSynchronize(b->st.s);
+    for (hipEvent_t e : b->sb.ev) (void)hipEventDestroy(e);
+    delete b;
+    return DTC_OK;
+}
+
+int dtc_solve_batch(const dtc_domain_desc* d, int64_t n_dom, const dtc_solve_opts* o, dtc_domain_result* res) {
+    dtc_batch* b = nullptr;
+    DTC_TRY(dtc_batch_create(d, n_dom, o, &b));
+    int rc = dtc_batch_run(b, res);
+    dtc_batch_destroy(b);
+    return rc;
+}
+
+int dtc_solve_domain(const dtc_domain_desc* d, const dtc_solve_opts* o, dtc_domain_result* res, double* u_out,
+                     double* r_out) {
+    dtc_batch* b = nullptr;
+    DTC_TRY(dtc_batch_create(d, 1, o, &b));
+    int rc = dtc_batch_run(b, res);
+    if (rc == DTC_OK && (u_out || r_out) && (res->status == DTC_OK || res->status == DTC_ERR_NO_CONVERGENCE))
+        rc = batch_fields(b->sb, 0, u_out, r_out);
+    dtc_batch_destroy(b);
+    return rc;
+}
+
+int dtc_rotation_search(const double EE[36], int criterion, double step_deg, double num_zero, double* alpha, double n[3],
+                        double aa[9], double EE_opt[36], int64_t* jj, double* crit) {
+    if (!EE || !alpha || !n || !aa || !EE_opt) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL argument");
+    if (criterion != 1 && criterion != 2) return set_error(DTC_ERR_BAD_ARGUMENT, "criterion %d", criterion);
+    DTC_TRY(check_device());
+    Stream st;
+    DTC_TRY(st.create());
+    std::vector<RotResult> rr;
+    DTC_TRY(rotation_search_batch(EE, 1, step_deg, num_zero > 0.0 ? num_zero : 1e-9, st.s, rr));
+    const RotResult& q = rr[criterion - 1];
+    *alpha = q.alpha;
+    std::memcpy(n, q.n, sizeof(q.n));
+    std::memcpy(aa, q.aa, sizeof(q.aa));
+    std::memcpy(EE_opt, q.EE_opt, sizeof(q.EE_opt));
+    if (jj) *jj = q.jj;
+    if (crit) *crit = q.crit;
+    return DTC_OK;
+}
+
+}  // extern "C"

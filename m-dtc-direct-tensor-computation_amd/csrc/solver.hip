@@ -1,0 +1,1271 @@
+// solver.hip -- batched matrix-free Jacobi-PCG for the 24 load cases of HEX08
+// voxel domains on gfx950.
+//
+// Replaces the PETSc path of f-src/mod_dtc_main_subroutines.f90:426-1238:
+// MatSetValues assembly of K_e = Hexe08(bone) per element, Dirichlet lifting
+// f = -A x_bc with f[bc] = 0, MatZeroRowsColumns(diag 0.0), KSPSolve with
+// CG + Jacobi and x0 = 0 for each load case, and the reactions r = A_org u.
+//
+// Design
+//  * Operator: never assembled.  Every element is a voxel with modulus E_v,
+//    so K_e = E_v * Khat.  In the signed Walsh basis of the 8 element nodes
+//    Khat is block diagonal (3x3, 2x2, 1x1 blocks, 45 coefficients), so one
+//    element product costs two 8-point Hadamard butterflies per direction
+//    plus 45 multiplies instead of a dense 24x24 product.
+//  * Tiling: the lattice of kept nodes is cut into bricks of 8x4x4 owned
+//    nodes.  A workgroup stages the halo (10x6x6 nodes) of its brick in LDS,
+//    computes every element touching an owned node (9x5x5 voxels) and
+//    accumulates into LDS in 8 parity classes (elements of one class share
+//    no node), then writes its owned nodes once.  No atomics, deterministic.
+//  * Vectors: node-major [node][3][24], nodes numbered brick by brick so a
+//    brick's owned nodes are contiguous.  All 24 load cases advance in
+//    lockstep with their own alpha/beta; a converged load case freezes.
+//  * Reductions: per-brick partial sums in FP64, then one workgroup per
+//    domain sums them in brick order (deterministic).
+//  * Precision: DTC_FP64 runs PCG in FP64; DTC_FP32_REFINE runs the inner
+//    PCG in FP32 on the residual equation and keeps x, the true residual
+//    r = -(K u)_free and the reactions in FP64.
+#include <cmath>
+#include <cstring>
+
+#include "bc.hpp"
+#include "solver.hpp"
+
+namespace dtc {
+
+namespace {
+
+constexpr int R = R24;
+constexpr int NT_UPD = 192;    // multiple of 24: a thread keeps one load case
+
+enum InMode { IN_V = 0, IN_U = 1 };
+enum OutMode { OUT_MASKED_DOT = 0, OUT_RESID = 1, OUT_PLAIN = 2 };
+
+template <typename T> struct WK { T g[45]; };
+
+template <typename T> WK<T> cast_wk(const WalshK& w) {
+    WK<T> o;
+    for (int i = 0; i < 45; ++i) o.g[i] = (T)w.g[i];
+    return o;
+}
+
+__device__ inline bool is_bc(const DomInfo& D, int gx, int gy, int gz) {
+    return gx == 0 || gy == 0 || gz == 0 || gx == D.nx || gy == D.ny || gz == D.nz;
+}
+
+__device__ inline void node_coord(const DomInfo& D, int gx, int gy, int gz, double c[3]) {
+    c[0] = ((double)gx + D.xa1[0]) * D.delta[0];
+    c[1] = ((double)gy + D.xa1[1]) * D.delta[1];
+    c[2] = ((double)gz + D.xa1[2]) * D.delta[2];
+}
+
+// ---------------------------------------------------------------- element operator
+// modes after the forward butterflies: index bit0 <-> x, bit1 <-> y, bit2 <-> z
+template <typename T>
+__device__ inline void fht_fwd(T v[8]) {
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            if (!(b & s)) {
+                T a = v[b], c = v[b + s];
+                v[b] = a + c;
+                v[b + s] = c - a;
+            }
+}
+template <typename T>
+__device__ inline void fht_inv(T v[8]) {
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            if (!(b & s)) {
+                T a = v[b], c = v[b + s];
+                v[b] = a - c;
+                v[b + s] = a + c;
+            }
+}
+
+// y = Kw w on the (mode, direction) groups; w, y indexed [mode][dir]
+template <typename T>
+__device__ inline void walsh_apply(const WK<T>& k, const T w[8][3], T y[8][3]) {
+#define M3(o, i0, d0, i1, d1, i2, d2)                                                         \
+    {                                                                                         \
+        T a0 = w[i0][d0], a1 = w[i1][d1], a2 = w[i2][d2];                                     \
+        y[i0][d0] = k.g[o + 0] * a0 + k.g[o + 1] * a1 + k.g[o + 2] * a2;                      \
+        y[i1][d1] = k.g[o + 3] * a0 + k.g[o + 4] * a1 + k.g[o + 5] * a2;                      \
+        y[i2][d2] = k.g[o + 6] * a0 + k.g[o + 7] * a1 + k.g[o + 8] * a2;                      \
+    }
+#define M2(o, i0, d0, i1, d1)                                                                 \
+    {                                                                                         \
+        T a0 = w[i0][d0], a1 = w[i1][d1];                                                     \
+        y[i0][d0] = k.g[o + 0] * a0 + k.g[o + 1] * a1;                                        \
+        y[i1][d1] = k.g[o + 2] * a0 + k.g[o + 3] * a1;                                        \
+    }
+    y[0][0] = y[0][1] = y[0][2] = T(0);
+    M3(0, 1, 0, 2, 1, 4, 2)          // x.ux, y.uy, z.uz
+    M2(9, 1, 1, 2, 0)                // x.uy, y.ux
+    M2(13, 1, 2, 4, 0)               // x.uz, z.ux
+    M2(17, 2, 2, 4, 1)               // y.uz, z.uy
+    M2(21, 3, 0, 6, 2)               // xy.ux, yz.uz
+    M2(25, 3, 1, 5, 2)               // xy.uy, xz.uz
+    M3(29, 3, 2, 5, 1, 6, 0)         // xy.uz, xz.uy, yz.ux
+    M2(38, 5, 0, 6, 1)               // xz.ux, yz.uy
+    y[7][0] = k.g[42] * w[7][0];     // xyz
+    y[7][1] = k.g[43] * w[7][1];
+    y[7][2] = k.g[44] * w[7][2];
+#undef M3
+#undef M2
+}
+
+// ---------------------------------------------------------------- matvec
+template <typename T> struct MvCfg;
+template <> struct MvCfg<float> { static constexpr int RC = 8, NT = 256, PAD = 4; };
+template <> struct MvCfg<double> { static constexpr int RC = 4, NT = 128, PAD = 2; };
+
+struct MvArgs {
+    const BrickInfo* bricks;
+    const DomInfo* doms;
+    const int* lat2node;
+    const double* emod;
+    const int* done;       // may be null
+    double* partial;       // [brick][24]
+};
+
+template <typename T, int IN, int OUT>
+__global__ __launch_bounds__(MvCfg<T>::NT) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
+                                                        T* __restrict__ yout) {
+    constexpr int RC = MvCfg<T>::RC, NT = MvCfg<T>::NT;
+    constexpr int SX = 3 * RC + MvCfg<T>::PAD;   // LDS node stride
+    constexpr int NH = HX * HY * HZ, NE = EX * EY * EZ, NO = BX * BY * BZ;
+    __shared__ int s_node[NH];
+    __shared__ unsigned char s_bc[NH];
+    __shared__ T s_E[NE];
+    __shared__ T s_x[NH * SX];
+    __shared__ T s_y[NO * SX];
+    __shared__ double s_red[NT];
+
+    const BrickInfo B = A.bricks[blockIdx.x];
+    if (A.done && A.done[B.dom]) return;
+    const DomInfo D = A.doms[B.dom];
+    const int tid = threadIdx.x;
+    const int Lx = D.nx + 1, Ly = D.ny + 1, Lz = D.nz + 1;
+
+    for (int i = tid; i < NH; i += NT) {
+        int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+        int gx = B.x0 - 1 + hx, gy = B.y0 - 1 + hy, gz = B.z0 - 1 + hz;
+        int q = -1;
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < Lx && gy < Ly && gz < Lz)
+            q = A.lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
+        s_node[i] = q;
+        s_bc[i] = (q >= 0 && is_bc(D, gx, gy, gz)) ? 1 : 0;
+    }
+    for (int i = tid; i < NE; i += NT) {
+        int ex = i % EX, ey = (i / EX) % EY, ez = i / (EX * EY);
+        int vx = B.x0 - 1 + ex, vy = B.y0 - 1 + ey, vz = B.z0 - 1 + ez;
+        T e = T(0);
+        if (vx >= 0 && vy >= 0 && vz >= 0 && vx < D.nx && vy < D.ny && vz < D.nz)
+            e = (T)A.emod[D.vox_off + vx + (int64_t)D.nx * (vy + (int64_t)D.ny * vz)];
+        s_E[i] = e;
+    }
+    __syncthreads();
+
+    for (int chunk = 0; chunk < R / RC; ++chunk) {
+        // ---- stage the input tile
+        for (int i = tid; i < NH * 3 * RC; i += NT) {
+            int h = i / (3 * RC), rem = i % (3 * RC), d = rem / RC, r = rem % RC;
+            int lc = chunk * RC + r;
+            int q = s_node[h];
+            T v = T(0);
+            if (q >= 0) {
+                if (IN == IN_V) {
+                    v = xin[((int64_t)q * 3 + d) * R + lc];
+                } else {
+                    if (s_bc[h]) {
+                        int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+                        double c[3], N[8];
+                        node_coord(D, B.x0 - 1 + hx, B.y0 - 1 + hy, B.z0 - 1 + hz, c);
+                        macro_shape8(c, D.min_c, D.max_c, N);
+                        v = (T)bc_value(N, d, lc);
+                    } else if (xin) {
+                        v = xin[((int64_t)q * 3 + d) * R + lc];
+                    }
+                }
+            }
+            s_x[h * SX + d * RC + r] = v;
+        }
+        for (int i = tid; i < NO * SX; i += NT) s_y[i] = T(0);
+        __syncthreads();
+
+        // ---- element products in 8 parity classes
+        for (int cls = 0; cls < 8; ++cls) {
+            const int px = cls & 1, py = (cls >> 1) & 1, pz = cls >> 2;
+            const int nex = (EX - px + 1) / 2, ney = (EY - py + 1) / 2, nez = (EZ - pz + 1) / 2;
+            const int ntask = nex * ney * nez * RC;
+            for (int t = tid; t < ntask; t += NT) {
+                const int r = t % RC, e = t / RC;
+                const int ex = 2 * (e % nex) + px, ey = 2 * ((e / nex) % ney) + py, ez = 2 * (e / (nex * ney)) + pz;
+                const T E = s_E[ex + EX * (ey + EY * ez)];
+                if (E == T(0)) continue;
+                T w[8][3], y[8][3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    T v[8];
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        int h = (ex + (b & 1)) + HX * ((ey + ((b >> 1) & 1)) + HY * (ez + (b >> 2)));
+                        v[b] = s_x[h * SX + d * RC + r];
+                    }
+                    fht_fwd(v);
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) w[b][d] = v[b];
+                }
+                walsh_apply(wk, w, y);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    T v[8];
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) v[b] = y[b][d] * E;
+                    fht_inv(v);
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        int hx = ex + (b & 1), hy = ey + ((b >> 1) & 1), hz = ez + (b >> 2);
+                        if (hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ) {
+                            int o = (hx - 1) + BX * ((hy - 1) + BY * (hz - 1));
+                            s_y[o * SX + d * RC + r] += v[b];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- write owned nodes
+        double acc = 0.0;
+        for (int i = tid; i < NO * 3 * RC; i += NT) {
+            int o = i / (3 * RC), rem = i % (3 * RC), d = rem / RC, r = rem % RC;
+            int ox = o % BX, oy = (o / BX) % BY, oz = o / (BX * BY);
+            int h = (ox + 1) + HX * ((oy + 1) + HY * (oz + 1));
+            int q = s_node[h];
+            if (q < 0) continue;
+            T y = s_y[o * SX + d * RC + r];
+            if (OUT == OUT_MASKED_DOT) {
+                if (s_bc[h]) y = T(0);
+                acc += (double)s_x[h * SX + d * RC + r] * (double)y;
+            } else if (OUT == OUT_RESID) {
+                y = s_bc[h] ? T(0) : -y;
+                acc += (double)y * (double)y;
+            }
+            yout[((int64_t)q * 3 + d) * R + chunk * RC + r] = y;
+        }
+        if (OUT != OUT_PLAIN) {
+            s_red[tid] = acc;
+            __syncthreads();
+            if (tid < RC) {
+                double s = 0.0;
+                for (int k = tid; k < NT; k += RC) s += s_red[k];
+                A.partial[(int64_t)blockIdx.x * R + chunk * RC + tid] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- vector kernels
+// per brick, thread t keeps load case t % 24
+template <typename T>
+__global__ __launch_bounds__(NT_UPD) void k_cg_init(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                                    const int* __restrict__ active, T* __restrict__ x,
+                                                    const T* __restrict__ r, T* __restrict__ p,
+                                                    const T* __restrict__ dinv, double* __restrict__ prz,
+                                                    double* __restrict__ prr, int zero_x) {
+    __shared__ double s1[NT_UPD], s2[NT_UPD];
+    const BrickInfo B = bricks[blockIdx.x];
+    const int tid = threadIdx.x, lc = tid % R;
+    double a1 = 0.0, a2 = 0.0;
+    if (!(done && done[B.dom])) {
+        const int act = active[B.dom * R + lc];
+        const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
+        for (int64_t i = tid; i < cnt; i += NT_UPD) {
+            int64_t idx = base + i;
+            int64_t node = idx / (3 * R);
+            T rv = r[idx];
+            T z = act ? rv * dinv[node] : T(0);
+            if (act) p[idx] = z;
+            if (zero_x) x[idx] = T(0);
+            a1 += (double)rv * (double)z;
+            a2 += (double)rv * (double)rv;
+        }
+    }
+    s1[tid] = a1; s2[tid] = a2;
+    __syncthreads();
+    if (tid < R) {
+        double u = 0.0, v = 0.0;
+        for (int k = tid; k < NT_UPD; k += R) { u += s1[k]; v += s2[k]; }
+        prz[(int64_t)blockIdx.x * R + tid] = u;
+        prr[(int64_t)blockIdx.x * R + tid] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT_UPD) void k_cg_update(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                                      const double* __restrict__ alpha, T* __restrict__ x,
+                                                      T* __restrict__ r, const T* __restrict__ p,
+                                                      const T* __restrict__ q, const T* __restrict__ dinv,
+                                                      double* __restrict__ prz, double* __restrict__ prr) {
+    __shared__ double s1[NT_UPD], s2[NT_UPD];
+    const BrickInfo B = bricks[blockIdx.x];
+    const int tid = threadIdx.x, lc = tid % R;
+    if (done && done[B.dom]) return;
+    const T a = (T)alpha[B.dom * R + lc];
+    double a1 = 0.0, a2 = 0.0;
+    const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
+    for (int64_t i = tid; i < cnt; i += NT_UPD) {
+        int64_t idx = base + i;
+        int64_t node = idx / (3 * R);
+        T xv = x[idx] + a * p[idx];
+        T rv = r[idx] - a * q[idx];
+        x[idx] = xv;
+        r[idx] = rv;
+        T z = rv * dinv[node];
+        a1 += (double)rv * (double)z;
+        a2 += (double)rv * (double)rv;
+    }
+    s1[tid] = a1; s2[tid] = a2;
+    __syncthreads();
+    if (tid < R) {
+        double u = 0.0, v = 0.0;
+        for (int k = tid; k < NT_UPD; k += R) { u += s1[k]; v += s2[k]; }
+        prz[(int64_t)blockIdx.x * R + tid] = u;
+        prr[(int64_t)blockIdx.x * R + tid] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT_UPD) void k_cg_dir(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                                   const double* __restrict__ beta, const T* __restrict__ r,
+                                                   T* __restrict__ p, const T* __restrict__ dinv) {
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done && done[B.dom]) return;
+    const int tid = threadIdx.x, lc = tid % R;
+    const T b = (T)beta[B.dom * R + lc];
+    const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
+    for (int64_t i = tid; i < cnt; i += NT_UPD) {
+        int64_t idx = base + i;
+        int64_t node = idx / (3 * R);
+        p[idx] = r[idx] * dinv[node] + b * p[idx];
+    }
+}
+
+// ---------------------------------------------------------------- per-domain reductions (one block per domain)
+__device__ inline double sum_partials(const double* part, int b0, int b1, int lc) {
+    double s = 0.0;
+    for (int b = b0; b < b1; ++b) s += part[(int64_t)b * R + lc];
+    return s;
+}
+
+__global__ void k_red_init(const DomInfo* __restrict__ doms, const double* __restrict__ prz,
+                           const double* __restrict__ prr, const double* __restrict__ thr2,
+                           int* __restrict__ active, double* __restrict__ rz, int* __restrict__ done) {
+    const int d = blockIdx.x, lc = threadIdx.x;
+    __shared__ int any;
+    if (lc == 0) any = 0;
+    __syncthreads();
+    if (lc < R) {
+        const DomInfo D = doms[d];
+        double z = sum_partials(prz, D.brick_begin, D.brick_end, lc);
+        double rr = sum_partials(prr, D.brick_begin, D.brick_end, lc);
+        int a = active[d * R + lc];
+        if (a && !(rr > thr2[d * R + lc])) a = 0;
+        active[d * R + lc] = a;
+        rz[d * R + lc] = z;
+        if (a) any = 1;
+    }
+    __syncthreads();
+    if (lc == 0) done[d] = any ? 0 : 1;
+}
+
+__global__ void k_red_alpha(const DomInfo* __restrict__ doms, const int* __restrict__ done,
+                            const double* __restrict__ ppq, int* __restrict__ active,
+                            const double* __restrict__ rz, double* __restrict__ alpha) {
+    const int d = blockIdx.x, lc = threadIdx.x;
+    if (lc >= R || done[d]) return;
+    const DomInfo D = doms[d];
+    double pq = sum_partials(ppq, D.brick_begin, D.brick_end, lc);
+    int a = active[d * R + lc];
+    double al = 0.0;
+    if (a) {
+        if (pq > 0.0) al = rz[d * R + lc] / pq;
+        else active[d * R + lc] = 0;      // breakdown: freeze this load case
+    }
+    alpha[d * R + lc] = al;
+}
+
+__global__ void k_red_beta(const DomInfo* __restrict__ doms, int* __restrict__ done,
+                           const double* __restrict__ prz, const double* __restrict__ prr,
+                           const double* __restrict__ thr2, int* __restrict__ active, double* __restrict__ rz,
+                           double* __restrict__ beta, int64_t* __restrict__ iters, int64_t max_iters) {
+    const int d = blockIdx.x, lc = threadIdx.x;
+    __shared__ int any;
+    if (done[d]) return;
+    if (lc == 0) any = 0;
+    __syncthreads();
+    if (lc < R) {
+        const DomInfo D = doms[d];
+        double z = sum_partials(prz, D.brick_begin, D.brick_end, lc);
+        double rr = sum_partials(prr, D.brick_begin, D.brick_end, lc);
+        int a = active[d * R + lc];
+        double b = 0.0;
+        if (a) {
+            if (!(rr > thr2[d * R + lc])) a = 0;
+            else {
+                double old = rz[d * R + lc];
+                b = old != 0.0 ? z / old : 0.0;
+            }
+        }
+        active[d * R + lc] = a;
+        rz[d * R + lc] = z;
+        beta[d * R + lc] = b;
+        if (a) any = 1;
+    }
+    __syncthreads();
+    if (lc == 0) {
+        int64_t it = iters[d] + 1;
+        iters[d] = it;
+        if (!any || it >= max_iters) done[d] = 1;
+    }
+}
+
+// outer (FP64) residual bookkeeping after a RESID pass: rr = ||r||^2 per load case
+__global__ void k_red_outer(const DomInfo* __restrict__ doms, const double* __restrict__ prr, int first,
+                            double tol2, double inner_tol2, int precision, double* __restrict__ rr0,
+                            double* __restrict__ rrcur, int* __restrict__ active64, int* __restrict__ active,
+                            double* __restrict__ scale, double* __restrict__ thr2, int* __restrict__ done) {
+    const int d = blockIdx.x, lc = threadIdx.x;
+    __shared__ int any;
+    if (lc == 0) any = 0;
+    __syncthreads();
+    if (lc < R) {
+        const DomInfo D = doms[d];
+        double rr = sum_partials(prr, D.brick_begin, D.brick_end, lc);
+        if (first) rr0[d * R + lc] = rr;
+        double r0 = rr0[d * R + lc];
+        rrcur[d * R + lc] = rr;
+        int a = (r0 > 0.0) && (rr > tol2 * r0);
+        active64[d * R + lc] = a;
+        active[d * R + lc] = a;
+        double s = a ? sqrt(rr) : 0.0;
+        scale[d * R + lc] = s;
+        // FP32: relative reduction of the normalised residual; FP64: the
+        // outer target directly, with a safety factor against drift
+        thr2[d * R + lc] = precision == DTC_FP32_REFINE ? inner_tol2 : 0.25 * tol2 * r0;
+        if (a) any = 1;
+    }
+    __syncthreads();
+    if (lc == 0) done[d] = any ? 0 : 1;
+}
+
+// r32 = r64 / s (normalised residual), x32 = 0
+__global__ __launch_bounds__(NT_UPD) void k_to32(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                                 const double* __restrict__ scale, const double* __restrict__ r64,
+                                                 float* __restrict__ r32, float* __restrict__ x32) {
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done[B.dom]) return;
+    const int tid = threadIdx.x, lc = tid % R;
+    const double s = scale[B.dom * R + lc];
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
+    for (int64_t i = tid; i < cnt; i += NT_UPD) {
+        r32[base + i] = (float)(r64[base + i] * inv);
+        x32[base + i] = 0.f;
+    }
+}
+
+// x64 += s * x32
+__global__ __launch_bounds__(NT_UPD) void k_acc64(const BrickInfo* __restrict__ bricks, const int* __restrict__ skip,
+                                                  const double* __restrict__ scale, const float* __restrict__ x32,
+                                                  double* __restrict__ x64) {
+    const BrickInfo B = bricks[blockIdx.x];
+    if (skip[B.dom]) return;
+    const int tid = threadIdx.x, lc = tid % R;
+    const double s = scale[B.dom * R + lc];
+    if (s == 0.0) return;   // uniform per thread; no barrier below
+    const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
+    for (int64_t i = tid; i < cnt; i += NT_UPD) x64[base + i] += s * (double)x32[base + i];
+}
+
+__global__ void k_fill_f64(double* __restrict__ a, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_i32b(int* __restrict__ a, int64_t n, int v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_i64(int64_t* __restrict__ a, int64_t n, int64_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------- topology
+__global__ void k_brick_count(const int* __restrict__ numLat, const int* __restrict__ keepN, int Lx, int Ly,
+                              int Lz, int bnx, int bny, int* __restrict__ cnt) {
+    __shared__ int s;
+    const int b = blockIdx.x, t = threadIdx.x;     // blockDim = BX*BY*BZ
+    if (t == 0) s = 0;
+    __syncthreads();
+    const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
+    const int gx = bx * BX + t % BX, gy = by * BY + (t / BX) % BY, gz = bz * BZ + t / (BX * BY);
+    if (gx < Lx && gy < Ly && gz < Lz) {
+        int num = numLat[gx + Lx * (gy + Ly * gz)];
+        if (num > 0 && keepN[num - 1]) atomicAdd(&s, 1);
+    }
+    __syncthreads();
+    if (t == 0) cnt[b] = s;
+}
+
+__global__ void k_brick_assign(const BrickInfo* __restrict__ bl, int brick0, const int* __restrict__ numLat,
+                               const int* __restrict__ keepN, const int* __restrict__ cref, int Lx, int Ly, int Lz,
+                               int64_t lat_off, int* __restrict__ lat2node, int* __restrict__ sol2ref) {
+    constexpr int NO = BX * BY * BZ;
+    __shared__ int flag[NO];
+    const BrickInfo B = bl[brick0 + blockIdx.x];
+    const int t = threadIdx.x;
+    const int gx = B.x0 + t % BX, gy = B.y0 + (t / BX) % BY, gz = B.z0 + t / (BX * BY);
+    int kept = 0, num = 0;
+    if (gx < Lx && gy < Ly && gz < Lz) {
+        num = numLat[gx + Lx * (gy + Ly * gz)];
+        kept = (num > 0 && keepN[num - 1]) ? 1 : 0;
+    }
+    flag[t] = kept;
+    __syncthreads();
+    if (kept) {
+        int rank = 0;
+        for (int k = 0; k < t; ++k) rank += flag[k];   // lexicographic order inside the brick
+        int s = B.node_begin + rank;
+        lat2node[lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)] = s;
+        sol2ref[s] = cref[num - 1] + 1;
+    }
+}
+
+__global__ void k_count_bc(const int* __restrict__ numLat, const int* __restrict__ keepN, int Lx, int Ly, int Lz,
+                           int* __restrict__ out) {
+    __shared__ int c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < (int64_t)Lx * Ly * Lz) {
+        int x = p % Lx, y = (p / Lx) % Ly, z = p / ((int64_t)Lx * Ly);
+        int num = numLat[p];
+        if (num > 0 && keepN[num - 1] && (x == 0 || y == 0 || z == 0 || x == Lx - 1 || y == Ly - 1 || z == Lz - 1))
+            atomicAdd(&c, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && c) atomicAdd(out, c);
+}
+
+// element modulus of every voxel; escale folds a domain's edge length a into
+// the modulus (K_e is linear in a), so one Walsh table serves the batch
+__global__ void k_emod(const int* __restrict__ keepE, const double* __restrict__ modulus, double E, double escale,
+                       int64_t nvox, double* __restrict__ emod) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    emod[v] = keepE[v] ? (modulus ? modulus[v] : E) * escale : 0.0;
+}
+
+// Jacobi: diag(K)_node,d = kdiag * sum of the moduli of the adjacent elements
+__global__ void k_dinv(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
+                       const int* __restrict__ lat2node, const double* __restrict__ emod, double kdiag,
+                       double* __restrict__ d64, float* __restrict__ d32) {
+    const BrickInfo B = bricks[blockIdx.x];
+    const DomInfo D = doms[B.dom];
+    const int t = threadIdx.x;
+    const int gx = B.x0 + t % BX, gy = B.y0 + (t / BX) % BY, gz = B.z0 + t / (BX * BY);
+    const int Lx = D.nx + 1, Ly = D.ny + 1, Lz = D.nz + 1;
+    if (gx >= Lx || gy >= Ly || gz >= Lz) return;
+    int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
+    if (q < 0) return;
+    double s = 0.0;
+    for (int dz = -1; dz <= 0; ++dz)
+        for (int dy = -1; dy <= 0; ++dy)
+            for (int dx = -1; dx <= 0; ++dx) {
+                int vx = gx + dx, vy = gy + dy, vz = gz + dz;
+                if (vx < 0 || vy < 0 || vz < 0 || vx >= D.nx || vy >= D.ny || vz >= D.nz) continue;
+                s += emod[D.vox_off + vx + (int64_t)D.nx * (vy + (int64_t)D.ny * vz)];
+            }
+    double dg = kdiag * s;
+    double inv = dg != 0.0 ? 1.0 / dg : 1.0;   // PCJACOBI uses 1 for a zero diagonal
+    if (is_bc(D, gx, gy, gz)) inv = 0.0;        // constrained DOFs stay zero
+    d64[q] = inv;
+    d32[q] = (float)inv;
+}
+
+// ---------------------------------------------------------------- force projection
+// ff(d*8+l, lc) = sum over owned nodes r_d(node, lc) N_l(xi(node))
+// (f-src/mod_struct_calcmat.f90:254-280); partial per brick [brick][lc][24]
+__global__ __launch_bounds__(NT_UPD) void k_project(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
+                                                    const int* __restrict__ lat2node, const double* __restrict__ rf,
+                                                    double* __restrict__ pff) {
+    constexpr int G = NT_UPD / R;    // 8 node groups
+    __shared__ double s[G][R][24];
+    const BrickInfo B = bricks[blockIdx.x];
+    const DomInfo D = doms[B.dom];
+    const int tid = threadIdx.x, lc = tid % R, g = tid / R;
+    const int Lx = D.nx + 1, Ly = D.ny + 1;
+    double acc[24];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) acc[k] = 0.0;
+    for (int o = g; o < BX * BY * BZ; o += G) {
+        int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
+        if (gx > D.nx || gy > D.ny || gz > D.nz) continue;
+        int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
+        if (q < 0) continue;
+        double c[3], N[8];
+        node_coord(D, gx, gy, gz, c);
+        macro_shape8(c, D.min_c, D.max_c, N);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double rv = rf[((int64_t)q * 3 + d) * R + lc];
+#pragma unroll
+            for (int l = 0; l < 8; ++l) acc[d * 8 + l] += rv * N[l];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 24; ++k) s[g][lc][k] = acc[k];
+    __syncthreads();
+    for (int i = tid; i < R * 24; i += NT_UPD) {
+        int l2 = i / 24, k = i % 24;
+        double v = 0.0;
+        for (int gg = 0; gg < G; ++gg) v += s[gg][l2][k];
+        pff[(int64_t)blockIdx.x * R * 24 + i] = v;
+    }
+}
+
+__global__ void k_red_ff(const DomInfo* __restrict__ doms, const double* __restrict__ pff, double* __restrict__ ff) {
+    const int d = blockIdx.x;
+    const DomInfo D = doms[d];
+    for (int i = threadIdx.x; i < R * 24; i += blockDim.x) {
+        double v = 0.0;
+        for (int b = D.brick_begin; b < D.brick_end; ++b) v += pff[(int64_t)b * R * 24 + i];
+        ff[(int64_t)d * R * 24 + i] = v;
+    }
+}
+
+// u (mesh numbering) = x + x_bc, column-major (3N, 24)
+__global__ __launch_bounds__(NT_UPD) void k_fields(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
+                                                   const int* __restrict__ lat2node, const int* __restrict__ sol2ref,
+                                                   const double* __restrict__ x, const double* __restrict__ rf,
+                                                   int dom, int64_t nn, double* __restrict__ uo,
+                                                   double* __restrict__ ro) {
+    const BrickInfo B = bricks[blockIdx.x];
+    if (B.dom != dom) return;
+    const DomInfo D = doms[B.dom];
+    const int Lx = D.nx + 1, Ly = D.ny + 1;
+    for (int i = threadIdx.x; i < BX * BY * BZ * 3 * R; i += NT_UPD) {
+        int o = i / (3 * R), rem = i % (3 * R), d = rem / R, lc = rem % R;
+        int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
+        if (gx > D.nx || gy > D.ny || gz > D.nz) continue;
+        int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
+        if (q < 0) continue;
+        int64_t ref = sol2ref[q] - 1;
+        int64_t out = (3 * ref + d) + 3 * nn * lc;
+        double v = x[((int64_t)q * 3 + d) * R + lc];
+        if (is_bc(D, gx, gy, gz)) {
+            double c[3], N[8];
+            node_coord(D, gx, gy, gz, c);
+            macro_shape8(c, D.min_c, D.max_c, N);
+            v = bc_value(N, d, lc);
+        }
+        if (uo) uo[out] = v;
+        if (ro) ro[out] = rf[((int64_t)q * 3 + d) * R + lc];
+    }
+}
+
+// ---------------------------------------------------------------- sensitivity
+// h[node][j][d] = sum_lc M[lc][j] u_lc(node, d), u = x + x_bc
+__global__ __launch_bounds__(NT_UPD) void k_grad_h(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
+                                                   const int* __restrict__ lat2node, const double* __restrict__ x,
+                                                   const double* __restrict__ Mj, int dom, double* __restrict__ h) {
+    const BrickInfo B = bricks[blockIdx.x];
+    if (B.dom != dom) return;
+    const DomInfo D = doms[B.dom];
+    const int Lx = D.nx + 1, Ly = D.ny + 1;
+    for (int i = threadIdx.x; i < BX * BY * BZ * 18; i += NT_UPD) {
+        int o = i / 18, j = (i % 18) / 3, d = i % 3;
+        int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
+        if (gx > D.nx || gy > D.ny || gz > D.nz) continue;
+        int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
+        if (q < 0) continue;
+        bool bc = is_bc(D, gx, gy, gz);
+        double N[8];
+        if (bc) {
+            double c[3];
+            node_coord(D, gx, gy, gz, c);
+            macro_shape8(c, D.min_c, D.max_c, N);
+        }
+        double s = 0.0;
+        for (int lc = 0; lc < R; ++lc) {
+            double u = bc ? bc_value(N, d, lc) : x[((int64_t)q * 3 + d) * R + lc];
+            s += Mj[lc * 6 + j] * u;
+        }
+        h[((int64_t)q * 6 + j) * 3 + d] = s;
+    }
+}
+
+__device__ inline double u_at(const DomInfo& D, const int* lat2node, const double* x, int gx, int gy, int gz, int d,
+                              int lc) {
+    const int Lx = D.nx + 1, Ly = D.ny + 1;
+    int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
+    if (q < 0) return 0.0;
+    if (is_bc(D, gx, gy, gz)) {
+        double c[3], N[8];
+        node_coord(D, gx, gy, gz, c);
+        macro_shape8(c, D.min_c, D.max_c, N);
+        return bc_value(N, d, lc);
+    }
+    return x[((int64_t)q * 3 + d) * R + lc];
+}
+
+// dL/dE_v = sum_j h_j,e^T Khat u_j,e for every voxel of domain dom
+__global__ void k_grad_elem(const DomInfo* __restrict__ doms, const int* __restrict__ lat2node,
+                            const double* __restrict__ emod, const double* __restrict__ x,
+                            const double* __restrict__ h, WK<double> wk, int dom, double* __restrict__ g) {
+    const DomInfo D = doms[dom];
+    const int64_t nvox = (int64_t)D.nx * D.ny * D.nz;
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    double out = 0.0;
+    if (emod[D.vox_off + v] != 0.0) {
+        const int vx = v % D.nx, vy = (v / D.nx) % D.ny, vz = v / ((int64_t)D.nx * D.ny);
+        const int Lx = D.nx + 1, Ly = D.ny + 1;
+        int qb[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            qb[b] = lat2node[D.lat_off + (vx + (b & 1)) + (int64_t)Lx * ((vy + ((b >> 1) & 1)) + (int64_t)Ly * (vz + (b >> 2)))];
+        for (int j = 0; j < 6; ++j) {
+            double w[8][3], y[8][3];
+            for (int d = 0; d < 3; ++d) {
+                double t[8];
+                for (int b = 0; b < 8; ++b)
+                    t[b] = u_at(D, lat2node, x, vx + (b & 1), vy + ((b >> 1) & 1), vz + (b >> 2), d, j);
+                fht_fwd(t);
+                for (int b = 0; b < 8; ++b) w[b][d] = t[b];
+            }
+            walsh_apply(wk, w, y);
+            for (int d = 0; d < 3; ++d) {
+                double t[8];
+                for (int b = 0; b < 8; ++b) t[b] = y[b][d];
+                fht_inv(t);
+                for (int b = 0; b < 8; ++b)
+                    if (qb[b] >= 0) out += t[b] * h[((int64_t)qb[b] * 6 + j) * 3 + d];
+            }
+        }
+    }
+    g[v] = out;
+}
+
+inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+int element_stiffness(int nen, double E, double nu, double a, double* K) {
+    if (nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "element stiffness: only HEX08 is implemented here");
+    if (!(a > 0.0) || !(nu > -1.0 && nu < 0.5)) return set_error(DTC_ERR_BAD_ARGUMENT, "element stiffness: a=%g nu=%g", a, nu);
+    const int nd = 24;
+    double D[6][6] = {};
+    const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) D[i][j] = lam;
+    for (int i = 0; i < 3; ++i) D[i][i] += 2.0 * mu;
+    D[3][3] = D[4][4] = D[5][5] = mu;
+    std::memset(K, 0, sizeof(double) * nd * nd);
+    const double g = 1.0 / std::sqrt(3.0), jac = 2.0 / a, detJ = (a / 2.0) * (a / 2.0) * (a / 2.0);
+    for (int qi = 0; qi < 8; ++qi) {
+        const double q[3] = {(qi & 1) ? g : -g, (qi & 2) ? g : -g, (qi & 4) ? g : -g};
+        double B[6][24] = {};
+        for (int l = 0; l < 8; ++l) {
+            const double sx = 2.0 * h08_off(l, 0) - 1, sy = 2.0 * h08_off(l, 1) - 1, sz = 2.0 * h08_off(l, 2) - 1;
+            const double dx = sx * (1 + sy * q[1]) * (1 + sz * q[2]) / 8.0 * jac;
+            const double dy = (1 + sx * q[0]) * sy * (1 + sz * q[2]) / 8.0 * jac;
+            const double dz = (1 + sx * q[0]) * (1 + sy * q[1]) * sz / 8.0 * jac;
+            B[0][3 * l] = dx; B[1][3 * l + 1] = dy; B[2][3 * l + 2] = dz;
+            B[3][3 * l] = dy; B[3][3 * l + 1] = dx;
+            B[4][3 * l] = dz; B[4][3 * l + 2] = dx;
+            B[5][3 * l + 1] = dz; B[5][3 * l + 2] = dy;
+        }
+        for (int r = 0; r < nd; ++r)
+            for (int c = 0; c < nd; ++c) {
+                double s = 0.0;
+                for (int m = 0; m < 6; ++m)
+                    for (int p = 0; p < 6; ++p) s += B[m][r] * D[m][p] * B[p][c];
+                K[r + nd * c] += detJ * s;
+            }
+    }
+    return DTC_OK;
+}
+
+int walsh_blocks(const double* Khat, WalshK* out) {
+    // node l <-> bit index b = ox + 2 oy + 4 oz; H[m][b] = prod over set bits i of m of (2 bit_i(b) - 1)
+    int l_of_b[8];
+    for (int l = 0; l < 8; ++l) l_of_b[h08_off(l, 0) + 2 * h08_off(l, 1) + 4 * h08_off(l, 2)] = l;
+    auto H = [](int m, int b) {
+        double s = 1.0;
+        for (int i = 0; i < 3; ++i)
+            if (m & (1 << i)) s *= ((b >> i) & 1) ? 1.0 : -1.0;
+        return s;
+    };
+    // Kw[(m,d),(m',d')] = sum H[m][b] Khat[(l(b),d),(l(b'),d')] H[m'][b'] / 64
+    static double Kw[24][24];
+    double kmax = 0.0;
+    for (int m = 0; m < 8; ++m)
+        for (int d = 0; d < 3; ++d)
+            for (int m2 = 0; m2 < 8; ++m2)
+                for (int d2 = 0; d2 < 3; ++d2) {
+                    double s = 0.0;
+                    for (int b = 0; b < 8; ++b)
+                        for (int b2 = 0; b2 < 8; ++b2)
+                            s += H(m, b) * Khat[(3 * l_of_b[b] + d) + 24 * (3 * l_of_b[b2] + d2)] * H(m2, b2);
+                    Kw[3 * m + d][3 * m2 + d2] = s / 64.0;
+                    kmax = std::fmax(kmax, std::fabs(s / 64.0));
+                }
+    // groups in the order walsh_apply reads them
+    const int groups[11][3][2] = {
+        {{1, 0}, {2, 1}, {4, 2}}, {{1, 1}, {2, 0}, {-1, 0}}, {{1, 2}, {4, 0}, {-1, 0}}, {{2, 2}, {4, 1}, {-1, 0}},
+        {{3, 0}, {6, 2}, {-1, 0}}, {{3, 1}, {5, 2}, {-1, 0}}, {{3, 2}, {5, 1}, {6, 0}}, {{5, 0}, {6, 1}, {-1, 0}},
+        {{7, 0}, {-1, 0}, {-1, 0}}, {{7, 1}, {-1, 0}, {-1, 0}}, {{7, 2}, {-1, 0}, {-1, 0}}};
+    int grp_of[24];
+    for (int i = 0; i < 24; ++i) grp_of[i] = -1;
+    int o = 0;
+    for (int g = 0; g < 11; ++g) {
+        int n = 0;
+        while (n < 3 && groups[g][n][0] >= 0) ++n;
+        for (int i = 0; i < n; ++i) grp_of[3 * groups[g][i][0] + groups[g][i][1]] = g;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                out->g[o++] = Kw[3 * groups[g][i][0] + groups[g][i][1]][3 * groups[g][j][0] + groups[g][j][1]];
+    }
+    if (o != 45) return set_error(DTC_ERR_DEVICE, "walsh_blocks: %d coefficients", o);
+    for (int i = 0; i < 24; ++i)
+        for (int j = 0; j < 24; ++j)
+            if ((grp_of[i] < 0 || grp_of[i] != grp_of[j]) && std::fabs(Kw[i][j]) > 1e-12 * kmax)
+                return set_error(DTC_ERR_UNSUPPORTED, "element stiffness is not block diagonal in the Walsh basis");
+    return DTC_OK;
+}
+
+int batch_begin(SolverBatch& b, const dtc_domain_desc* d, int64_t n_dom) {
+    int64_t nlat = 0, nvox = 0;
+    for (int64_t i = 0; i < n_dom; ++i) {
+        nlat += (d[i].n[0] + 1) * (d[i].n[1] + 1) * (d[i].n[2] + 1);
+        nvox += d[i].n[0] * d[i].n[1] * d[i].n[2];
+    }
+    DTC_TRY(b.lat2node.ensure((size_t)nlat * 4));
+    DTC_TRY(b.sol2ref.ensure((size_t)nlat * 4));
+    DTC_TRY(b.emod.ensure((size_t)nvox * 8));
+    k_fill_i32b<<<nb(nlat, 256), 256, 0, b.stream>>>(b.lat2node.as<int>(), nlat, -1);
+    k_fill_f64<<<nb(nvox, 256), 256, 0, b.stream>>>(b.emod.as<double>(), nvox, 0.0);
+    DTC_HIP(hipGetLastError());
+    b.n_dom = 0; b.n_nodes = 0; b.n_lat = 0; b.n_vox = 0;
+    b.h_dom.clear(); b.h_brick.clear(); b.h_elems.clear(); b.h_nodes.clear(); b.h_bnodes.clear(); b.h_status.clear();
+    return DTC_OK;
+}
+
+// Empty or failed domains keep a DomInfo with no bricks so indices stay aligned.
+int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status) {
+    DomInfo D{};
+    D.nx = (int)d.n[0]; D.ny = (int)d.n[1]; D.nz = (int)d.n[2];
+    D.brick_begin = D.brick_end = (int)b.h_brick.size();
+    D.node_begin = D.node_end = b.n_nodes;
+    D.lat_off = b.n_lat; D.vox_off = b.n_vox;
+    b.h_dom.push_back(D);
+    b.n_lat += (d.n[0] + 1) * (d.n[1] + 1) * (d.n[2] + 1);
+    b.n_vox += d.n[0] * d.n[1] * d.n[2];
+    b.n_dom += 1;
+    b.h_elems.push_back(0);
+    b.h_nodes.push_back(0);
+    b.h_bnodes.push_back(0);
+    b.h_status.push_back(status);
+    return DTC_OK;
+}
+
+int batch_add_domain(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d, const double* modulus_dev,
+                     double escale, hipStream_t s) {
+    if (m.nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "the solver implements HEX08 micro elements");
+    DomInfo D{};
+    D.nx = m.nx; D.ny = m.ny; D.nz = m.nz;
+    const int Lx = m.Lx, Ly = m.Ly, Lz = m.Lz;
+    D.bnx = (Lx + BX - 1) / BX; D.bny = (Ly + BY - 1) / BY; D.bnz = (Lz + BZ - 1) / BZ;
+    const int nbr = D.bnx * D.bny * D.bnz;
+    for (int k = 0; k < 3; ++k) {
+        D.delta[k] = d.delta[k];
+        D.xa1[k] = (double)(d.xa_n[k] - 1);
+        D.min_c[k] = (double)(d.xa_n[k] - 1) * d.delta[k];
+        D.max_c[k] = (double)(d.xa_n[k] + d.n[k] - 1) * d.delta[k];
+    }
+    {   // boundary nodes: fewer than 6 constrained DOFs fails the domain (f-src/mod_struct_preprocess.f90:471-473)
+        DevMem nbc;
+        DTC_TRY(nbc.alloc(4));
+        int zero = 0, hb = 0;
+        DTC_HIP(hipMemcpyAsync(nbc.p, &zero, 4, hipMemcpyHostToDevice, s));
+        const int64_t nl = (int64_t)Lx * Ly * Lz;
+        k_count_bc<<<nb(nl, 256), 256, 0, s>>>(m.numLat.as<int>(), m.keepN.as<int>(), Lx, Ly, Lz, nbc.as<int>());
+        DTC_HIP(hipGetLastError());
+        DTC_HIP(hipMemcpyAsync(&hb, nbc.p, 4, hipMemcpyDeviceToHost, s));
+        DTC_HIP(hipStreamSynchronize(s));
+        if (3 * hb < 6) {
+            DTC_TRY(batch_add_empty(b, d, DTC_ERR_FEW_CONSTRAINTS));
+            b.h_elems.back() = m.n_elems;
+            b.h_nodes.back() = m.n_nodes;
+            b.h_bnodes.back() = hb;
+            return DTC_OK;
+        }
+        b.h_bnodes.push_back(hb);
+    }
+    DevMem cnt;
+    DTC_TRY(cnt.alloc((size_t)nbr * 4));
+    k_brick_count<<<nbr, BX * BY * BZ, 0, s>>>(m.numLat.as<int>(), m.keepN.as<int>(), Lx, Ly, Lz, D.bnx, D.bny, cnt.as<int>());
+    DTC_HIP(hipGetLastError());
+    std::vector<int> hc(nbr);
+    DTC_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)nbr * 4, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipStreamSynchronize(s));
+    D.brick_begin = (int)b.h_brick.size();
+    D.node_begin = b.n_nodes;
+    int64_t node = b.n_nodes;
+    std::vector<BrickInfo> mine;
+    for (int i = 0; i < nbr; ++i) {
+        if (!hc[i]) continue;
+        BrickInfo B{};
+        B.dom = b.n_dom;
+        B.x0 = (i % D.bnx) * BX; B.y0 = ((i / D.bnx) % D.bny) * BY; B.z0 = (i / (D.bnx * D.bny)) * BZ;
+        B.node_begin = (int)node;
+        node += hc[i];
+        B.node_end = (int)node;
+        mine.push_back(B);
+    }
+    if (node >= INT32_MAX / 72) return set_error(DTC_ERR_BAD_ARGUMENT, "batch exceeds %d solver nodes", INT32_MAX / 72);
+    D.brick_end = D.brick_begin + (int)mine.size();
+    D.node_end = node;
+    D.lat_off = b.n_lat;
+    D.vox_off = b.n_vox;
+    const int64_t nvox = (int64_t)m.nx * m.ny * m.nz;
+    if (!mine.empty()) {
+        DevMem bl;
+        DTC_TRY(bl.alloc(mine.size() * sizeof(BrickInfo)));
+        DTC_HIP(hipMemcpyAsync(bl.p, mine.data(), mine.size() * sizeof(BrickInfo), hipMemcpyHostToDevice, s));
+        k_brick_assign<<<(unsigned)mine.size(), BX * BY * BZ, 0, s>>>(bl.as<BrickInfo>(), 0, m.numLat.as<int>(), m.keepN.as<int>(),
+                                                                   m.cref.as<int>(), Lx, Ly, Lz, D.lat_off, b.lat2node.as<int>(),
+                                                                   b.sol2ref.as<int>());
+        DTC_HIP(hipGetLastError());
+        DTC_HIP(hipStreamSynchronize(s));   // bl goes out of scope
+    }
+    k_emod<<<nb(nvox, 256), 256, 0, s>>>(m.keepE.as<int>(), modulus_dev, d.E, escale, nvox, b.emod.as<double>() + D.vox_off);
+    DTC_HIP(hipGetLastError());
+    DTC_HIP(hipStreamSynchronize(s));
+    b.h_brick.insert(b.h_brick.end(), mine.begin(), mine.end());
+    b.h_dom.push_back(D);
+    b.n_nodes = node;
+    b.n_lat += (int64_t)Lx * Ly * Lz;
+    b.n_vox += nvox;
+    b.n_dom += 1;
+    b.h_elems.push_back(m.n_elems);
+    b.h_nodes.push_back(m.n_nodes);
+    b.h_status.push_back(DTC_OK);
+    return DTC_OK;
+}
+
+int batch_finalize(SolverBatch& b, int precision) {
+    b.n_bricks = (int)b.h_brick.size();
+    hipStream_t s = b.stream;
+    DTC_TRY(b.dom.ensure(b.h_dom.size() * sizeof(DomInfo)));
+    DTC_HIP(hipMemcpyAsync(b.dom.p, b.h_dom.data(), b.h_dom.size() * sizeof(DomInfo), hipMemcpyHostToDevice, s));
+    DTC_TRY(b.brick.ensure((b.h_brick.size() + 1) * sizeof(BrickInfo)));
+    if (!b.h_brick.empty())
+        DTC_HIP(hipMemcpyAsync(b.brick.p, b.h_brick.data(), b.h_brick.size() * sizeof(BrickInfo), hipMemcpyHostToDevice, s));
+    const int64_t nn = b.n_nodes > 0 ? b.n_nodes : 1;
+    const size_t v64 = (size_t)nn * 3 * R * 8, v32 = (size_t)nn * 3 * R * 4;
+    DTC_TRY(b.dinv64.ensure((size_t)nn * 8));
+    DTC_TRY(b.dinv32.ensure((size_t)nn * 4));
+    DTC_TRY(b.x64.ensure(v64));
+    DTC_TRY(b.r64.ensure(v64));
+    if (precision == DTC_FP64) {
+        DTC_TRY(b.p64.ensure(v64));
+        DTC_TRY(b.q64.ensure(v64));
+    } else {
+        DTC_TRY(b.x32.ensure(v32));
+        DTC_TRY(b.r32.ensure(v32));
+        DTC_TRY(b.p32.ensure(v32));
+        DTC_TRY(b.q32.ensure(v32));
+    }
+    const size_t nbk = (size_t)(b.n_bricks > 0 ? b.n_bricks : 1);
+    DTC_TRY(b.part_a.ensure(nbk * R * 8));
+    DTC_TRY(b.part_b.ensure(nbk * R * 8));
+    DTC_TRY(b.part_c.ensure(nbk * R * 8));
+    DTC_TRY(b.part_ff.ensure(nbk * R * 24 * 8));
+    const size_t nd = (size_t)(b.n_dom > 0 ? b.n_dom : 1);
+    for (DevMem* m : {&b.alpha, &b.beta, &b.rz, &b.rr, &b.rr0, &b.thr2, &b.scale}) DTC_TRY(m->alloc(nd * R * 8));
+    DTC_TRY(b.active.ensure(nd * R * 4));
+    DTC_TRY(b.active64.ensure(nd * R * 4));
+    DTC_TRY(b.done.ensure(nd * 4));
+    DTC_TRY(b.done64.ensure(nd * 4));
+    DTC_TRY(b.iters.ensure(nd * 8));
+    DTC_TRY(b.ff.ensure(nd * R * 24 * 8));
+    // element stiffness (E = 1) and its Walsh blocks
+    double Khat[576];
+    DTC_TRY(element_stiffness(8, 1.0, b.nu, b.a, Khat));
+    DTC_TRY(walsh_blocks(Khat, &b.wk64));
+    b.kdiag = Khat[0];
+    if (b.n_bricks > 0) {
+        k_dinv<<<b.n_bricks, BX * BY * BZ, 0, s>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
+                                                   b.emod.as<double>(), b.kdiag, b.dinv64.as<double>(), b.dinv32.as<float>());
+        DTC_HIP(hipGetLastError());
+    }
+    DTC_HIP(hipStreamSynchronize(s));
+    b.precision = precision;
+    return DTC_OK;
+}
+
+namespace {
+
+int event_pair(SolverBatch& b, hipEvent_t* e0, hipEvent_t* e1) {
+    if (b.ev_free.size() < 2) {
+        for (int k = 0; k < 2; ++k) {
+            hipEvent_t e;
+            DTC_HIP(hipEventCreate(&e));
+            b.ev.push_back(e);
+            b.ev_free.push_back(e);
+        }
+    }
+    *e1 = b.ev_free.back(); b.ev_free.pop_back();
+    *e0 = b.ev_free.back(); b.ev_free.pop_back();
+    b.ev_used.push_back(*e0);
+    b.ev_used.push_back(*e1);
+    return DTC_OK;
+}
+
+// after a stream synchronisation: accumulate the timed launches
+int harvest_events(SolverBatch& b) {
+    for (size_t k = 0; k + 1 < b.ev_used.size(); k += 2) {
+        float ms = 0.f;
+        DTC_HIP(hipEventElapsedTime(&ms, b.ev_used[k], b.ev_used[k + 1]));
+        b.mv_ms_sum += ms;
+        b.mv_count += 1;
+        b.ev_free.push_back(b.ev_used[k]);
+        b.ev_free.push_back(b.ev_used[k + 1]);
+    }
+    b.ev_used.clear();
+    return DTC_OK;
+}
+
+template <typename T, int IN, int OUT>
+int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* part, const int* done) {
+    if (b.n_bricks == 0) return DTC_OK;
+    MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(), b.emod.as<double>(), done, part};
+    // the CG matvec is the dominant kernel: bracket it with events on the batch stream
+    const bool t = b.timing && IN == IN_V && OUT == OUT_MASKED_DOT;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (t) {
+        DTC_TRY(event_pair(b, &e0, &e1));
+        DTC_HIP(hipEventRecord(e0, b.stream));
+    }
+    k_matvec<T, IN, OUT><<<b.n_bricks, MvCfg<T>::NT, 0, b.stream>>>(a, wk, xin, yout);
+    DTC_HIP(hipGetLastError());
+    if (t) DTC_HIP(hipEventRecord(e1, b.stream));
+    return DTC_OK;
+}
+
+// one PCG iteration on (x, r, p, q) in precision T
+template <typename T>
+int pcg_iteration(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* dinv, int64_t max_iters) {
+    const int D = b.n_dom, NBk = b.n_bricks;
+    int* done = b.done.as<int>();
+    DTC_TRY((launch_mv<T, IN_V, OUT_MASKED_DOT>(b, wk, p, q, b.part_a.as<double>(), done)));
+    k_red_alpha<<<D, 64, 0, b.stream>>>(b.dom.as<DomInfo>(), done, b.part_a.as<double>(), b.active.as<int>(),
+                                        b.rz.as<double>(), b.alpha.as<double>());
+    k_cg_update<T><<<NBk, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), done, b.alpha.as<double>(), x, r, p, q, dinv,
+                                                 b.part_a.as<double>(), b.part_b.as<double>());
+    k_red_beta<<<D, 64, 0, b.stream>>>(b.dom.as<DomInfo>(), done, b.part_a.as<double>(), b.part_b.as<double>(),
+                                       b.thr2.as<double>(), b.active.as<int>(), b.rz.as<double>(), b.beta.as<double>(),
+                                       b.iters.as<int64_t>(), max_iters);
+    k_cg_dir<T><<<NBk, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), done, b.beta.as<double>(), r, p, dinv);
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+int all_done(SolverBatch& b, const DevMem& flags, bool* out) {
+    std::vector<int> h(b.n_dom);
+    DTC_HIP(hipMemcpyAsync(h.data(), flags.p, (size_t)b.n_dom * 4, hipMemcpyDeviceToHost, b.stream));
+    DTC_HIP(hipStreamSynchronize(b.stream));
+    DTC_TRY(harvest_events(b));
+    bool all = true;
+    for (int v : h) all = all && (v != 0);
+    *out = all;
+    return DTC_OK;
+}
+
+template <typename T>
+int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* dinv, int64_t max_iters) {
+    constexpr int CHECK = 20;
+    for (;;) {
+        bool fin = false;
+        DTC_TRY(all_done(b, b.done, &fin));
+        if (fin) return DTC_OK;
+        for (int k = 0; k < CHECK; ++k) DTC_TRY(pcg_iteration<T>(b, wk, x, r, p, q, dinv, max_iters));
+    }
+}
+
+// FP64 true residual r = -(K u)_free of u = x64 + x_bc (x64 == null: u = x_bc)
+int residual64(SolverBatch& b, const WK<double>& wk, const double* x64, int first, const dtc_solve_opts& o) {
+    DTC_TRY((launch_mv<double, IN_U, OUT_RESID>(b, wk, x64, b.r64.as<double>(), b.part_c.as<double>(), nullptr)));
+    k_red_outer<<<b.n_dom, 64, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_c.as<double>(), first, o.rel_tol * o.rel_tol,
+                                              o.inner_tol * o.inner_tol, b.precision, b.rr0.as<double>(),
+                                              b.rr.as<double>(), b.active64.as<int>(), b.active.as<int>(),
+                                              b.scale.as<double>(), b.thr2.as<double>(), b.done64.as<int>());
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+}  // namespace
+
+int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& iters, std::vector<int64_t>& outer,
+                std::vector<double>& relres) {
+    const int D = b.n_dom;
+    iters.assign(D, 0);
+    outer.assign(D, 0);
+    relres.assign(D, 0.0);
+    if (D == 0 || b.n_bricks == 0) return DTC_OK;
+    hipStream_t s = b.stream;
+    const int64_t nv = b.n_nodes * 3 * R;
+    const WK<double> wk64 = cast_wk<double>(b.wk64);
+    const WK<float> wk32 = cast_wk<float>(b.wk64);
+    const int64_t cap = o.max_iter > 0 ? o.max_iter : 100000;
+    k_fill_i64<<<nb(D, 64), 64, 0, s>>>(b.iters.as<int64_t>(), D, 0);
+    k_fill_f64<<<nb(nv, 256), 256, 0, s>>>(b.x64.as<double>(), nv, 0.0);
+    DTC_HIP(hipGetLastError());
+    b.mv_ms_sum = 0.0;
+    b.mv_count = 0;
+    DTC_TRY(residual64(b, wk64, nullptr, 1, o));
+    std::vector<int64_t> nouter(D, 0);
+    for (int it = 0; it < 64; ++it) {
+        bool fin = false;
+        DTC_TRY(all_done(b, b.done64, &fin));
+        if (fin) break;
+        {   // count outer rounds per domain and stop domains past the iteration cap
+            std::vector<int> h(D);
+            std::vector<int64_t> hi(D);
+            DTC_HIP(hipMemcpyAsync(h.data(), b.done64.p, (size_t)D * 4, hipMemcpyDeviceToHost, s));
+            DTC_HIP(hipMemcpyAsync(hi.data(), b.iters.p, (size_t)D * 8, hipMemcpyDeviceToHost, s));
+            DTC_HIP(hipStreamSynchronize(s));
+            bool any = false;
+            for (int d = 0; d < D; ++d) {
+                if (!h[d] && hi[d] >= cap) { h[d] = 1; }
+                if (!h[d]) { nouter[d] += 1; any = true; }
+            }
+            if (!any) break;
+            DTC_HIP(hipMemcpyAsync(b.done64.p, h.data(), (size_t)D * 4, hipMemcpyHostToDevice, s));
+            DTC_HIP(hipMemcpyAsync(b.done.p, h.data(), (size_t)D * 4, hipMemcpyHostToDevice, s));
+        }
+        if (b.precision == DTC_FP64) {
+            double* x = b.x64.as<double>();
+            double* r = b.r64.as<double>();
+            k_cg_init<double><<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), b.done64.as<int>(), b.active.as<int>(), x, r,
+                                                            b.p64.as<double>(), b.dinv64.as<double>(), b.part_a.as<double>(),
+                                                            b.part_b.as<double>(), 0);
+            k_red_init<<<D, 64, 0, s>>>(b.dom.as<DomInfo>(), b.part_a.as<double>(), b.part_b.as<double>(), b.thr2.as<double>(),
+                                        b.active.as<int>(), b.rz.as<double>(), b.done.as<int>());
+            DTC_HIP(hipGetLastError());
+            DTC_TRY(pcg_run<double>(b, wk64, x, r, b.p64.as<double>(), b.q64.as<double>(), b.dinv64.as<double>(), cap));
+        } else {
+            k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), b.done64.as<int>(), b.scale.as<double>(),
+                                                 b.r64.as<double>(), b.r32.as<float>(), b.x32.as<float>());
+            k_cg_init<float><<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), b.done64.as<int>(), b.active.as<int>(),
+                                                           b.x32.as<float>(), b.r32.as<float>(), b.p32.as<float>(),
+                                                           b.dinv32.as<float>(), b.part_a.as<double>(), b.part_b.as<double>(), 0);
+            k_red_init<<<D, 64, 0, s>>>(b.dom.as<DomInfo>(), b.part_a.as<double>(), b.part_b.as<double>(), b.thr2.as<double>(),
+                                        b.active.as<int>(), b.rz.as<double>(), b.done.as<int>());
+            DTC_HIP(hipGetLastError());
+            DTC_TRY(pcg_run<float>(b, wk32, b.x32.as<float>(), b.r32.as<float>(), b.p32.as<float>(), b.q32.as<float>(),
+                                   b.dinv32.as<float>(), cap));
+            k_acc64<<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), b.done64.as<int>(), b.scale.as<double>(),
+                                                  b.x32.as<float>(), b.x64.as<double>());
+            DTC_HIP(hipGetLastError());
+        }
+        DTC_TRY(residual64(b, wk64, b.x64.as<double>(), 0, o));
+    }
+    // results: iterations, true relative residual, convergence status
+    std::vector<int64_t> hi(D);
+    std::vector<double> hrr(D * R), hr0(D * R);
+    DTC_HIP(hipMemcpyAsync(hi.data(), b.iters.p, (size_t)D * 8, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipMemcpyAsync(hrr.data(), b.rr.p, (size_t)D * R * 8, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipMemcpyAsync(hr0.data(), b.rr0.p, (size_t)D * R * 8, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipStreamSynchronize(s));
+    for (int d = 0; d < D; ++d) {
+        iters[d] = hi[d];
+        outer[d] = nouter[d];
+        double mx = 0.0;
+        for (int lc = 0; lc < R; ++lc)
+            if (hr0[d * R + lc] > 0.0) mx = std::fmax(mx, std::sqrt(hrr[d * R + lc] / hr0[d * R + lc]));
+        relres[d] = mx;
+        if (b.h_status[d] == DTC_OK && mx > o.rel_tol) b.h_status[d] = DTC_ERR_NO_CONVERGENCE;
+    }
+    return DTC_OK;
+}
+
+int batch_forces(SolverBatch& b, std::vector<double>& ff) {
+    const int D = b.n_dom;
+    ff.assign((size_t)D * R * 24, 0.0);
+    if (D == 0 || b.n_bricks == 0) return DTC_OK;
+    const WK<double> wk64 = cast_wk<double>(b.wk64);
+    // reactions on the unmodified operator: rf = K (x + x_bc), stored in r64
+    DTC_TRY((launch_mv<double, IN_U, OUT_PLAIN>(b, wk64, b.x64.as<double>(), b.r64.as<double>(), nullptr, nullptr)));
+    k_project<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
+                                                   b.r64.as<double>(), b.part_ff.as<double>());
+    k_red_ff<<<D, 256, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_ff.as<double>(), b.ff.as<double>());
+    DTC_HIP(hipGetLastError());
+    DTC_HIP(hipMemcpyAsync(ff.data(), b.ff.p, ff.size() * 8, hipMemcpyDeviceToHost, b.stream));
+    DTC_HIP(hipStreamSynchronize(b.stream));
+    b.reactions_valid = true;
+    return DTC_OK;
+}
+
+int batch_fields(SolverBatch& b, int i, double* u_out, double* r_out) {
+    if (i < 0 || i >= b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %d", i);
+    const int64_t nn = b.h_nodes[i];
+    if (nn == 0) return DTC_OK;
+    if (r_out && !b.reactions_valid) return set_error(DTC_ERR_BAD_ARGUMENT, "reactions requested before the force pass");
+    DevMem du, dr;
+    const size_t bytes = (size_t)nn * 3 * R * 8;
+    if (u_out) DTC_TRY(du.alloc(bytes));
+    if (r_out) DTC_TRY(dr.alloc(bytes));
+    k_fields<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
+                                                  b.sol2ref.as<int>(), b.x64.as<double>(), b.r64.as<double>(), i, nn,
+                                                  u_out ? du.as<double>() : nullptr, r_out ? dr.as<double>() : nullptr);
+    DTC_HIP(hipGetLastError());
+    if (u_out) DTC_HIP(hipMemcpyAsync(u_out, du.p, bytes, hipMemcpyDeviceToHost, b.stream));
+    if (r_out) DTC_HIP(hipMemcpyAsync(r_out, dr.p, bytes, hipMemcpyDeviceToHost, b.stream));
+    DTC_HIP(hipStreamSynchronize(b.stream));
+    return DTC_OK;
+}
+
+int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host) {
+    if (i < 0 || i >= b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %d", i);
+    const DomInfo& D = b.h_dom[i];
+    const int64_t nvox = (int64_t)D.nx * D.ny * D.nz;
+    const int64_t nn = b.n_nodes > 0 ? b.n_nodes : 1;
+    DevMem h, Md, g;
+    DTC_TRY(h.alloc((size_t)nn * 18 * 8));
+    DTC_TRY(Md.alloc(24 * 6 * 8));
+    DTC_TRY(g.alloc((size_t)nvox * 8));
+    DTC_HIP(hipMemcpyAsync(Md.p, M, 24 * 6 * 8, hipMemcpyHostToDevice, b.stream));
+    if (D.brick_end > D.brick_begin) {
+        k_grad_h<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
+                                                      b.x64.as<double>(), Md.as<double>(), i, h.as<double>());
+    }
+    k_grad_elem<<<nb(nvox, 128), 128, 0, b.stream>>>(b.dom.as<DomInfo>(), b.lat2node.as<int>(), b.emod.as<double>(),
+                                                     b.x64.as<double>(), h.as<double>(), cast_wk<double>(b.wk64), i,
+                                                     g.as<double>());
+    DTC_HIP(hipGetLastError());
+    DTC_HIP(hipMemcpyAsync(dL_dE_host, g.p, (size_t)nvox * 8, hipMemcpyDeviceToHost, b.stream));
+    DTC_HIP(hipStreamSynchronize(b.stream));
+    return DTC_OK;
+}
+
+}  // namespace dtc

@@ -1,0 +1,76 @@
+// solver.hpp -- batched matrix-free PCG over HEX08 voxel domains (internal).
+#pragma once
+#include "dtc_internal.hpp"
+
+namespace dtc {
+
+// One batch of domains resident on the device: topology, vectors, scalars.
+struct SolverBatch {
+    int n_dom = 0;
+    int n_bricks = 0;
+    int64_t n_nodes = 0;           // solver nodes over all domains
+    int64_t n_lat = 0, n_vox = 0;  // lattice map / voxel entries over all domains
+    std::vector<DomInfo> h_dom;
+    std::vector<BrickInfo> h_brick;
+    std::vector<int64_t> h_elems, h_nodes, h_bnodes;   // per domain counts
+    std::vector<int> h_status;
+    DevMem dom, brick;             // DomInfo[], BrickInfo[]
+    DevMem lat2node;               // int32 per lattice node (-1 = no solver node)
+    DevMem sol2ref;                // int32 per solver node: mesh node id (1-based)
+    DevMem emod;                   // double per voxel (0 = no element)
+    DevMem dinv64, dinv32;         // Jacobi: 1 / (kdiag * sum of adjacent moduli)
+    // vectors, layout [node][3][24]
+    DevMem x64, r64, p64, q64;
+    DevMem x32, r32, p32, q32;
+    // per-brick partial sums
+    DevMem part_a, part_b, part_c; // [brick][24]
+    DevMem part_ff;                // [brick][24 lc][24]
+    // per-domain scalars [dom][24]
+    DevMem alpha, beta, rz, rr, rr0, thr2, scale;
+    DevMem active, active64;       // int [dom][24]
+    DevMem done;                   // int [dom]: inner iteration finished
+    DevMem done64;                 // int [dom]: outer (FP64 residual) converged
+    DevMem iters;                  // int64 [dom]
+    DevMem ff;                     // double [dom][24 lc][24]
+    DevMem gtmp;                   // gradient work
+    WalshK wk64{};
+    double kdiag = 0.0;            // diagonal entry of the E = 1 element stiffness
+    double nu = 0.3, a = 1.0;
+    int precision = DTC_FP64;
+    bool reactions_valid = false;
+    bool capturing = false;
+    hipStream_t stream = nullptr;
+    // matvec timing
+    std::vector<hipEvent_t> ev, ev_free, ev_used;
+    double mv_ms_sum = 0.0;
+    int64_t mv_count = 0;
+    bool timing = false;
+};
+
+// Walsh-basis block coefficients of the E = 1 HEX08 element stiffness.
+int walsh_blocks(const double* Khat, WalshK* out);
+// Host element stiffness by Gauss quadrature (column-major 3nen x 3nen).
+int element_stiffness(int nen, double E, double nu, double a, double* K);
+
+// Reserve the lattice map and voxel modulus arrays of n_dom domains.
+int batch_begin(SolverBatch& b, const dtc_domain_desc* d, int64_t n_dom);
+// Append one meshed domain (HEX08 micro) to the batch topology.  d.modulus:
+// modulus_dev: optional per-voxel moduli on the device (n^3), otherwise d.E;
+// escale multiplies every modulus.
+int batch_add_domain(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d, const double* modulus_dev,
+                     double escale, hipStream_t s);
+// Append a domain that failed before the solve (empty mesh, ...).
+int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status);
+// Allocate vectors and scalars once all domains are added.
+int batch_finalize(SolverBatch& b, int precision);
+// Full solve: returns per-domain status in b.h_status; fills iteration counts.
+int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& iters,
+                std::vector<int64_t>& outer, std::vector<double>& relres);
+// ff (24x24 per domain, [lc][dof]) after a solve.
+int batch_forces(SolverBatch& b, std::vector<double>& ff);
+// displacements / reactions of domain i in mesh numbering, column-major (3N, 24).
+int batch_fields(SolverBatch& b, int i, double* u_out, double* r_out);
+// sensitivity dL/dE per voxel of domain i.
+int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host);
+
+}  // namespace dtc
