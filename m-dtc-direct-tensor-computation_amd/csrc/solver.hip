@@ -119,9 +119,21 @@ __device__ inline void walsh_apply(const WK<T>& k, const T w[8][3], T y[8][3]) {
 }
 
 // ---------------------------------------------------------------- matvec
+// Load cases are processed in chunks of RC; one (node, direction) segment of
+// a chunk is 32 bytes = two 16-byte vectors (float4 / double2).
 template <typename T> struct MvCfg;
-template <> struct MvCfg<float> { static constexpr int RC = 8, NT = 256, PAD = 4; };
-template <> struct MvCfg<double> { static constexpr int RC = 4, NT = 128, PAD = 2; };
+template <> struct MvCfg<float> { static constexpr int RC = 8; using V = float4; };
+template <> struct MvCfg<double> { static constexpr int RC = 4; using V = double2; };
+constexpr int MV_NT = 256;
+constexpr int VPS = 2;          // 16-byte vectors per (node, direction) segment
+constexpr int VPN = 3 * VPS;    // per node and chunk
+
+__device__ inline float vcomp(const float4& v, int w) { return w == 0 ? v.x : w == 1 ? v.y : w == 2 ? v.z : v.w; }
+__device__ inline double vcomp(const double2& v, int w) { return w == 0 ? v.x : v.y; }
+__device__ inline void vzero(float4& v) { v = make_float4(0.f, 0.f, 0.f, 0.f); }
+__device__ inline void vzero(double2& v) { v = make_double2(0.0, 0.0); }
+__device__ inline float4 vneg(const float4& v) { return make_float4(-v.x, -v.y, -v.z, -v.w); }
+__device__ inline double2 vneg(const double2& v) { return make_double2(-v.x, -v.y); }
 
 struct MvArgs {
     const BrickInfo* bricks;
@@ -132,18 +144,72 @@ struct MvArgs {
     double* partial;       // [brick][24]
 };
 
+// Parity classes of the element tile: elements (ex,ey,ez) with the same
+// parities share no node, so one class accumulates into LDS without conflicts.
+// element products of one class (one task = solid element x load case)
+template <typename T>
+__device__ __forceinline__ void mv_class(int ncls, const short* __restrict__ elist, const WK<T>& wk,
+                                         const T* __restrict__ s_E, const T* __restrict__ s_x, T* __restrict__ s_y,
+                                         int tid) {
+    constexpr int RC = MvCfg<T>::RC, SX = 3 * RC;
+    const int ntask = ncls * RC;
+#pragma unroll 1
+    for (int t = tid; t < ntask; t += MV_NT) {
+        const int r = t % RC;
+        const int i = elist[t / RC];
+        const int ex = i % EX, ey = (i / EX) % EY, ez = i / (EX * EY);
+        const T E = s_E[i];
+        T w[8][3], y[8][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            T v[8];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int h = (ex + (b & 1)) + HX * ((ey + ((b >> 1) & 1)) + HY * (ez + (b >> 2)));
+                v[b] = s_x[h * SX + d * RC + r];
+            }
+            fht_fwd(v);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) w[b][d] = v[b];
+        }
+        walsh_apply(wk, w, y);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            T v[8];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) v[b] = y[b][d] * E;
+            fht_inv(v);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int hx = ex + (b & 1), hy = ey + ((b >> 1) & 1), hz = ez + (b >> 2);
+                if (hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ) {
+                    const int o = (hx - 1) + BX * ((hy - 1) + BY * (hz - 1));
+                    s_y[o * SX + d * RC + r] += v[b];
+                }
+            }
+        }
+    }
+}
+
 template <typename T, int IN, int OUT>
-__global__ __launch_bounds__(MvCfg<T>::NT) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
-                                                        T* __restrict__ yout) {
-    constexpr int RC = MvCfg<T>::RC, NT = MvCfg<T>::NT;
-    constexpr int SX = 3 * RC + MvCfg<T>::PAD;   // LDS node stride
+__global__ __launch_bounds__(MV_NT, 3) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
+                                                 T* __restrict__ yout) {
+    constexpr int RC = MvCfg<T>::RC, SX = 3 * RC, VW = 16 / (int)sizeof(T);
+    using V = typename MvCfg<T>::V;
     constexpr int NH = HX * HY * HZ, NE = EX * EY * EZ, NO = BX * BY * BZ;
+    constexpr int NLD = (NH * VPN + MV_NT - 1) / MV_NT;
+    constexpr int NOP = MV_NT / VPN;              // owned nodes per pass of the epilogue
     __shared__ int s_node[NH];
     __shared__ unsigned char s_bc[NH];
     __shared__ T s_E[NE];
-    __shared__ T s_x[NH * SX];
-    __shared__ T s_y[NO * SX];
-    __shared__ double s_red[NT];
+    constexpr int MAXC = ((EX + 1) / 2) * ((EY + 1) / 2) * ((EZ + 1) / 2);   // largest parity class
+    __shared__ short s_elist[8 * MAXC];           // solid elements of the tile, per parity class
+    __shared__ int s_ccount[8];
+    __shared__ __align__(16) T s_x[NH * SX];      // also the reduction scratch after the epilogue
+    __shared__ __align__(16) T s_y[NO * SX];
+    V* s_xv = reinterpret_cast<V*>(s_x);
+    V* s_yv = reinterpret_cast<V*>(s_y);
+    double* s_red = reinterpret_cast<double*>(s_x);
 
     const BrickInfo B = A.bricks[blockIdx.x];
     if (A.done && A.done[B.dom]) return;
@@ -151,38 +217,69 @@ __global__ __launch_bounds__(MvCfg<T>::NT) void k_matvec(MvArgs A, WK<T> wk, con
     const int tid = threadIdx.x;
     const int Lx = D.nx + 1, Ly = D.ny + 1, Lz = D.nz + 1;
 
-    for (int i = tid; i < NH; i += NT) {
-        int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
-        int gx = B.x0 - 1 + hx, gy = B.y0 - 1 + hy, gz = B.z0 - 1 + hz;
+    for (int i = tid; i < NH; i += MV_NT) {
+        const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+        const int gx = B.x0 - 1 + hx, gy = B.y0 - 1 + hy, gz = B.z0 - 1 + hz;
         int q = -1;
         if (gx >= 0 && gy >= 0 && gz >= 0 && gx < Lx && gy < Ly && gz < Lz)
             q = A.lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
         s_node[i] = q;
         s_bc[i] = (q >= 0 && is_bc(D, gx, gy, gz)) ? 1 : 0;
     }
-    for (int i = tid; i < NE; i += NT) {
-        int ex = i % EX, ey = (i / EX) % EY, ez = i / (EX * EY);
-        int vx = B.x0 - 1 + ex, vy = B.y0 - 1 + ey, vz = B.z0 - 1 + ez;
+    if (tid < 8) s_ccount[tid] = 0;
+    __syncthreads();
+    for (int i = tid; i < NE; i += MV_NT) {
+        const int ex = i % EX, ey = (i / EX) % EY, ez = i / (EX * EY);
+        const int vx = B.x0 - 1 + ex, vy = B.y0 - 1 + ey, vz = B.z0 - 1 + ez;
         T e = T(0);
         if (vx >= 0 && vy >= 0 && vz >= 0 && vx < D.nx && vy < D.ny && vz < D.nz)
             e = (T)A.emod[D.vox_off + vx + (int64_t)D.nx * (vy + (int64_t)D.ny * vz)];
         s_E[i] = e;
+        if (e != T(0)) {
+            // order inside a class does not matter: a class adds at most once to a node
+            const int c = (ex & 1) | ((ey & 1) << 1) | ((ez & 1) << 2);
+            s_elist[c * MAXC + atomicAdd(&s_ccount[c], 1)] = (short)i;
+        }
     }
     __syncthreads();
 
+    const V* xv = reinterpret_cast<const V*>(xin);
+    V* yv = reinterpret_cast<V*>(yout);
+    V reg[NLD];
+    // 16-byte loads of one chunk into registers (IN_V); issued before the element
+    // phase of the previous chunk so their latency overlaps the compute
+    auto prefetch = [&](int chunk) {
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            const int i = tid + k * MV_NT;
+            V v;
+            vzero(v);
+            if (i < NH * VPN) {
+                const int h = i / VPN, j = i % VPN, d = j / VPS, c = j % VPS;
+                const int q = s_node[h];
+                if (q >= 0) v = xv[(((int64_t)q * 3 + d) * R + chunk * RC) / VW + c];
+            }
+            reg[k] = v;
+        }
+    };
+    if (IN == IN_V) prefetch(0);
+
     for (int chunk = 0; chunk < R / RC; ++chunk) {
-        // ---- stage the input tile
-        for (int i = tid; i < NH * 3 * RC; i += NT) {
-            int h = i / (3 * RC), rem = i % (3 * RC), d = rem / RC, r = rem % RC;
-            int lc = chunk * RC + r;
-            int q = s_node[h];
-            T v = T(0);
-            if (q >= 0) {
-                if (IN == IN_V) {
-                    v = xin[((int64_t)q * 3 + d) * R + lc];
-                } else {
+        if (IN == IN_V) {
+#pragma unroll
+            for (int k = 0; k < NLD; ++k) {
+                const int i = tid + k * MV_NT;
+                if (i < NH * VPN) s_xv[i] = reg[k];
+            }
+        } else {
+            for (int i = tid; i < NH * SX; i += MV_NT) {
+                const int h = i / SX, rem = i % SX, d = rem / RC, r = rem % RC;
+                const int lc = chunk * RC + r;
+                const int q = s_node[h];
+                T v = T(0);
+                if (q >= 0) {
                     if (s_bc[h]) {
-                        int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+                        const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
                         double c[3], N[8];
                         node_coord(D, B.x0 - 1 + hx, B.y0 - 1 + hy, B.z0 - 1 + hz, c);
                         macro_shape8(c, D.min_c, D.max_c, N);
@@ -191,83 +288,60 @@ __global__ __launch_bounds__(MvCfg<T>::NT) void k_matvec(MvArgs A, WK<T> wk, con
                         v = xin[((int64_t)q * 3 + d) * R + lc];
                     }
                 }
+                s_x[i] = v;
             }
-            s_x[h * SX + d * RC + r] = v;
         }
-        for (int i = tid; i < NO * SX; i += NT) s_y[i] = T(0);
+        for (int i = tid; i < NO * VPN; i += MV_NT) vzero(s_yv[i]);
         __syncthreads();
+        if (IN == IN_V && chunk + 1 < R / RC) prefetch(chunk + 1);
 
-        // ---- element products in 8 parity classes
+#pragma unroll 1
         for (int cls = 0; cls < 8; ++cls) {
-            const int px = cls & 1, py = (cls >> 1) & 1, pz = cls >> 2;
-            const int nex = (EX - px + 1) / 2, ney = (EY - py + 1) / 2, nez = (EZ - pz + 1) / 2;
-            const int ntask = nex * ney * nez * RC;
-            for (int t = tid; t < ntask; t += NT) {
-                const int r = t % RC, e = t / RC;
-                const int ex = 2 * (e % nex) + px, ey = 2 * ((e / nex) % ney) + py, ez = 2 * (e / (nex * ney)) + pz;
-                const T E = s_E[ex + EX * (ey + EY * ez)];
-                if (E == T(0)) continue;
-                T w[8][3], y[8][3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    T v[8];
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        int h = (ex + (b & 1)) + HX * ((ey + ((b >> 1) & 1)) + HY * (ez + (b >> 2)));
-                        v[b] = s_x[h * SX + d * RC + r];
-                    }
-                    fht_fwd(v);
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) w[b][d] = v[b];
-                }
-                walsh_apply(wk, w, y);
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    T v[8];
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) v[b] = y[b][d] * E;
-                    fht_inv(v);
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        int hx = ex + (b & 1), hy = ey + ((b >> 1) & 1), hz = ez + (b >> 2);
-                        if (hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ) {
-                            int o = (hx - 1) + BX * ((hy - 1) + BY * (hz - 1));
-                            s_y[o * SX + d * RC + r] += v[b];
-                        }
-                    }
-                }
-            }
+            const int ncls = s_ccount[cls];
+            if (ncls == 0) continue;     // uniform across the block
+            mv_class<T>(ncls, s_elist + cls * MAXC, wk, s_E, s_x, s_y, tid);
             __syncthreads();
         }
 
-        // ---- write owned nodes
-        double acc = 0.0;
-        for (int i = tid; i < NO * 3 * RC; i += NT) {
-            int o = i / (3 * RC), rem = i % (3 * RC), d = rem / RC, r = rem % RC;
-            int ox = o % BX, oy = (o / BX) % BY, oz = o / (BX * BY);
-            int h = (ox + 1) + HX * ((oy + 1) + HY * (oz + 1));
-            int q = s_node[h];
-            if (q < 0) continue;
-            T y = s_y[o * SX + d * RC + r];
-            if (OUT == OUT_MASKED_DOT) {
-                if (s_bc[h]) y = T(0);
-                acc += (double)s_x[h * SX + d * RC + r] * (double)y;
-            } else if (OUT == OUT_RESID) {
-                y = s_bc[h] ? T(0) : -y;
-                acc += (double)y * (double)y;
+        // ---- write owned nodes: thread keeps one 16-byte column jj of the node segment
+        const int jj = tid % VPN, d = jj / VPS, c = jj % VPS;
+        double acc[VW];
+#pragma unroll
+        for (int w = 0; w < VW; ++w) acc[w] = 0.0;
+        if (tid < NOP * VPN) {
+            for (int o = tid / VPN; o < NO; o += NOP) {
+                const int h = (o % BX + 1) + HX * ((o / BX) % BY + 1) + HX * HY * (o / (BX * BY) + 1);
+                const int q = s_node[h];
+                if (q < 0) continue;
+                V y = s_yv[o * VPN + jj];
+                if (OUT == OUT_MASKED_DOT) {
+                    if (s_bc[h]) vzero(y);
+                    const V x = s_xv[h * VPN + jj];
+#pragma unroll
+                    for (int w = 0; w < VW; ++w) acc[w] += (double)vcomp(x, w) * (double)vcomp(y, w);
+                } else if (OUT == OUT_RESID) {
+                    if (s_bc[h]) vzero(y); else y = vneg(y);
+#pragma unroll
+                    for (int w = 0; w < VW; ++w) acc[w] += (double)vcomp(y, w) * (double)vcomp(y, w);
+                }
+                yv[(((int64_t)q * 3 + d) * R + chunk * RC) / VW + c] = y;
             }
-            yout[((int64_t)q * 3 + d) * R + chunk * RC + r] = y;
         }
+        __syncthreads();
         if (OUT != OUT_PLAIN) {
-            s_red[tid] = acc;
+#pragma unroll
+            for (int w = 0; w < VW; ++w) s_red[tid * VW + w] = acc[w];
             __syncthreads();
             if (tid < RC) {
-                double s = 0.0;
-                for (int k = tid; k < NT; k += RC) s += s_red[k];
-                A.partial[(int64_t)blockIdx.x * R + chunk * RC + tid] = s;
+                // load case r of the chunk lives in column c = r / VW, lane w = r % VW;
+                // threads with jj % VPS == c hold it (VPN is even, so tid % 2 == c)
+                const int cc = tid / VW, w = tid % VW;
+                double sum = 0.0;
+                for (int t = cc; t < NOP * VPN; t += VPS) sum += s_red[t * VW + w];
+                A.partial[(int64_t)blockIdx.x * R + chunk * RC + tid] = sum;
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -278,20 +352,24 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_init(const BrickInfo* __restrict_
                                                     const int* __restrict__ active, T* __restrict__ x,
                                                     const T* __restrict__ r, T* __restrict__ p,
                                                     const T* __restrict__ dinv, double* __restrict__ prz,
-                                                    double* __restrict__ prr, int zero_x) {
+                                                    double* __restrict__ prr, int zero_x, int set_p,
+                                                    const int* __restrict__ keep_p) {
     __shared__ double s1[NT_UPD], s2[NT_UPD];
     const BrickInfo B = bricks[blockIdx.x];
     const int tid = threadIdx.x, lc = tid % R;
     double a1 = 0.0, a2 = 0.0;
     if (!(done && done[B.dom])) {
         const int act = active[B.dom * R + lc];
+        // keep the search direction of a load case still iterating when its
+        // residual was replaced; restart it otherwise
+        const int sp = set_p || !(keep_p && keep_p[B.dom * R + lc]);
         const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
         for (int64_t i = tid; i < cnt; i += NT_UPD) {
             int64_t idx = base + i;
             int64_t node = idx / (3 * R);
             T rv = r[idx];
             T z = act ? rv * dinv[node] : T(0);
-            if (act) p[idx] = z;
+            if (act && sp) p[idx] = z;
             if (zero_x) x[idx] = T(0);
             a1 += (double)rv * (double)z;
             a2 += (double)rv * (double)rv;
@@ -323,10 +401,12 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_update(const BrickInfo* __restric
     for (int64_t i = tid; i < cnt; i += NT_UPD) {
         int64_t idx = base + i;
         int64_t node = idx / (3 * R);
-        T xv = x[idx] + a * p[idx];
-        T rv = r[idx] - a * q[idx];
-        x[idx] = xv;
-        r[idx] = rv;
+        T rv = r[idx];
+        if (a != T(0)) {      // frozen load cases keep x and r
+            rv -= a * q[idx];
+            x[idx] += a * p[idx];
+            r[idx] = rv;
+        }
         T z = rv * dinv[node];
         a1 += (double)rv * (double)z;
         a2 += (double)rv * (double)rv;
@@ -358,143 +438,172 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_dir(const BrickInfo* __restrict__
 }
 
 // ---------------------------------------------------------------- per-domain reductions (one block per domain)
-__device__ inline double sum_partials(const double* part, int b0, int b1, int lc) {
-    double s = 0.0;
-    for (int b = b0; b < b1; ++b) s += part[(int64_t)b * R + lc];
-    return s;
-}
+constexpr int RED_NT = 256;
 
-__global__ void k_red_init(const DomInfo* __restrict__ doms, const double* __restrict__ prz,
-                           const double* __restrict__ prr, const double* __restrict__ thr2,
-                           int* __restrict__ active, double* __restrict__ rz, int* __restrict__ done) {
-    const int d = blockIdx.x, lc = threadIdx.x;
-    __shared__ int any;
-    if (lc == 0) any = 0;
-    __syncthreads();
-    if (lc < R) {
-        const DomInfo D = doms[d];
-        double z = sum_partials(prz, D.brick_begin, D.brick_end, lc);
-        double rr = sum_partials(prr, D.brick_begin, D.brick_end, lc);
-        int a = active[d * R + lc];
-        if (a && !(rr > thr2[d * R + lc])) a = 0;
-        active[d * R + lc] = a;
-        rz[d * R + lc] = z;
-        if (a) any = 1;
+// sum over the bricks of a domain of part[b][lc] for the 24 load cases:
+// strided per thread, then a fixed shuffle/LDS tree (deterministic)
+__device__ inline void dom_sum24(const double* __restrict__ part, int b0, int b1, double* s_out /* [4][24] */,
+                                 double out[R]) {
+    const int tid = threadIdx.x;
+    double acc[R];
+#pragma unroll
+    for (int lc = 0; lc < R; ++lc) acc[lc] = 0.0;
+    for (int b = b0 + tid; b < b1; b += RED_NT)
+#pragma unroll
+        for (int lc = 0; lc < R; ++lc) acc[lc] += part[(int64_t)b * R + lc];
+#pragma unroll
+    for (int lc = 0; lc < R; ++lc) {
+        double v = acc[lc];
+        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+        acc[lc] = v;
     }
+    const int wave = tid / 64, lane = tid % 64;
+    if (lane == 0)
+        for (int lc = 0; lc < R; ++lc) s_out[wave * R + lc] = acc[lc];
     __syncthreads();
-    if (lc == 0) done[d] = any ? 0 : 1;
+    for (int lc = 0; lc < R; ++lc) out[lc] = ((s_out[lc] + s_out[R + lc]) + s_out[2 * R + lc]) + s_out[3 * R + lc];
+    __syncthreads();
 }
 
-__global__ void k_red_alpha(const DomInfo* __restrict__ doms, const int* __restrict__ done,
-                            const double* __restrict__ ppq, int* __restrict__ active,
-                            const double* __restrict__ rz, double* __restrict__ alpha) {
-    const int d = blockIdx.x, lc = threadIdx.x;
-    if (lc >= R || done[d]) return;
+__global__ __launch_bounds__(RED_NT) void k_red_init(const DomInfo* __restrict__ doms, const int* __restrict__ skip,
+                                                     const double* __restrict__ prz, const double* __restrict__ prr,
+                                                     const double* __restrict__ thr2, int* __restrict__ active,
+                                                     double* __restrict__ rz, int* __restrict__ done) {
+    __shared__ double s_out[4 * R];
+    const int d = blockIdx.x, tid = threadIdx.x;
     const DomInfo D = doms[d];
-    double pq = sum_partials(ppq, D.brick_begin, D.brick_end, lc);
-    int a = active[d * R + lc];
-    double al = 0.0;
-    if (a) {
-        if (pq > 0.0) al = rz[d * R + lc] / pq;
-        else active[d * R + lc] = 0;      // breakdown: freeze this load case
+    double z[R], rr[R];
+    dom_sum24(prz, D.brick_begin, D.brick_end, s_out, z);
+    dom_sum24(prr, D.brick_begin, D.brick_end, s_out, rr);
+    if (tid == 0) {
+        int any = 0;
+        for (int lc = 0; lc < R; ++lc) {
+            int a = (skip && skip[d]) ? 0 : active[d * R + lc];
+            if (a && !(rr[lc] > thr2[d * R + lc])) a = 0;
+            active[d * R + lc] = a;
+            rz[d * R + lc] = z[lc];
+            any |= a;
+        }
+        done[d] = any ? 0 : 1;
     }
-    alpha[d * R + lc] = al;
 }
 
-__global__ void k_red_beta(const DomInfo* __restrict__ doms, int* __restrict__ done,
-                           const double* __restrict__ prz, const double* __restrict__ prr,
-                           const double* __restrict__ thr2, int* __restrict__ active, double* __restrict__ rz,
-                           double* __restrict__ beta, int64_t* __restrict__ iters, int64_t max_iters) {
-    const int d = blockIdx.x, lc = threadIdx.x;
-    __shared__ int any;
+__global__ __launch_bounds__(RED_NT) void k_red_alpha(const DomInfo* __restrict__ doms, const int* __restrict__ done,
+                                                      const double* __restrict__ ppq, int* __restrict__ active,
+                                                      const double* __restrict__ rz, double* __restrict__ alpha) {
+    __shared__ double s_out[4 * R];
+    const int d = blockIdx.x, tid = threadIdx.x;
     if (done[d]) return;
-    if (lc == 0) any = 0;
-    __syncthreads();
-    if (lc < R) {
-        const DomInfo D = doms[d];
-        double z = sum_partials(prz, D.brick_begin, D.brick_end, lc);
-        double rr = sum_partials(prr, D.brick_begin, D.brick_end, lc);
+    const DomInfo D = doms[d];
+    double pq[R];
+    dom_sum24(ppq, D.brick_begin, D.brick_end, s_out, pq);
+    if (tid < R) {
+        const int lc = tid;
         int a = active[d * R + lc];
-        double b = 0.0;
+        double al = 0.0;
         if (a) {
-            if (!(rr > thr2[d * R + lc])) a = 0;
-            else {
-                double old = rz[d * R + lc];
-                b = old != 0.0 ? z / old : 0.0;
-            }
+            if (pq[lc] > 0.0) al = rz[d * R + lc] / pq[lc];
+            else active[d * R + lc] = 0;      // breakdown: freeze this load case
         }
-        active[d * R + lc] = a;
-        rz[d * R + lc] = z;
-        beta[d * R + lc] = b;
-        if (a) any = 1;
+        alpha[d * R + lc] = al;
     }
-    __syncthreads();
-    if (lc == 0) {
-        int64_t it = iters[d] + 1;
+}
+
+__global__ __launch_bounds__(RED_NT) void k_red_beta(const DomInfo* __restrict__ doms, int* __restrict__ done,
+                                                     const double* __restrict__ prz, const double* __restrict__ prr,
+                                                     const double* __restrict__ thr2, int* __restrict__ active,
+                                                     double* __restrict__ rz, double* __restrict__ beta,
+                                                     int64_t* __restrict__ iters, int64_t max_iters) {
+    __shared__ double s_out[4 * R];
+    const int d = blockIdx.x, tid = threadIdx.x;
+    if (done[d]) return;
+    const DomInfo D = doms[d];
+    double z[R], rr[R];
+    dom_sum24(prz, D.brick_begin, D.brick_end, s_out, z);
+    dom_sum24(prr, D.brick_begin, D.brick_end, s_out, rr);
+    if (tid == 0) {
+        int any = 0;
+        for (int lc = 0; lc < R; ++lc) {
+            int a = active[d * R + lc];
+            double bt = 0.0;
+            if (a) {
+                if (!(rr[lc] > thr2[d * R + lc])) a = 0;
+                else {
+                    const double old = rz[d * R + lc];
+                    bt = old != 0.0 ? z[lc] / old : 0.0;
+                }
+            }
+            active[d * R + lc] = a;
+            rz[d * R + lc] = z[lc];
+            beta[d * R + lc] = bt;
+            any |= a;
+        }
+        const int64_t it = iters[d] + 1;
         iters[d] = it;
         if (!any || it >= max_iters) done[d] = 1;
     }
 }
 
-// outer (FP64) residual bookkeeping after a RESID pass: rr = ||r||^2 per load case
-__global__ void k_red_outer(const DomInfo* __restrict__ doms, const double* __restrict__ prr, int first,
-                            double tol2, double inner_tol2, int precision, double* __restrict__ rr0,
-                            double* __restrict__ rrcur, int* __restrict__ active64, int* __restrict__ active,
-                            double* __restrict__ scale, double* __restrict__ thr2, int* __restrict__ done) {
-    const int d = blockIdx.x, lc = threadIdx.x;
-    __shared__ int any;
-    if (lc == 0) any = 0;
-    __syncthreads();
-    if (lc < R) {
-        const DomInfo D = doms[d];
-        double rr = sum_partials(prr, D.brick_begin, D.brick_end, lc);
-        if (first) rr0[d * R + lc] = rr;
-        double r0 = rr0[d * R + lc];
-        rrcur[d * R + lc] = rr;
-        int a = (r0 > 0.0) && (rr > tol2 * r0);
-        active64[d * R + lc] = a;
-        active[d * R + lc] = a;
-        double s = a ? sqrt(rr) : 0.0;
-        scale[d * R + lc] = s;
-        // FP32: relative reduction of the normalised residual; FP64: the
-        // outer target directly, with a safety factor against drift
-        thr2[d * R + lc] = precision == DTC_FP32_REFINE ? inner_tol2 : 0.25 * tol2 * r0;
-        if (a) any = 1;
+// outer (FP64) residual bookkeeping after a RESID pass: rr = ||f - K u||^2 per load case;
+// load cases above tol^2 ||f||^2 stay active; the recursive PCG residual
+// target is (tol/2)^2 ||f||^2 so the true residual check passes after drift
+__global__ __launch_bounds__(RED_NT) void k_red_outer(const DomInfo* __restrict__ doms, const double* __restrict__ prr,
+                                                      int first, double tol2, double* __restrict__ rr0,
+                                                      double* __restrict__ rrcur, int* __restrict__ active64,
+                                                      int* __restrict__ active, double* __restrict__ thr2,
+                                                      int* __restrict__ done64, int* __restrict__ done) {
+    __shared__ double s_out[4 * R];
+    const int d = blockIdx.x, tid = threadIdx.x;
+    const DomInfo D = doms[d];
+    double rr[R];
+    dom_sum24(prr, D.brick_begin, D.brick_end, s_out, rr);
+    if (tid == 0) {
+        int any = 0;
+        for (int lc = 0; lc < R; ++lc) {
+            if (first) rr0[d * R + lc] = rr[lc];
+            const double r0 = rr0[d * R + lc];
+            rrcur[d * R + lc] = rr[lc];
+            const int a = (r0 > 0.0) && (rr[lc] > tol2 * r0);
+            active64[d * R + lc] = a;
+            active[d * R + lc] = a;
+            thr2[d * R + lc] = 0.25 * tol2 * r0;
+            any |= a;
+        }
+        done64[d] = any ? 0 : 1;
+        done[d] = any ? 0 : 1;
     }
-    __syncthreads();
-    if (lc == 0) done[d] = any ? 0 : 1;
 }
 
-// r32 = r64 / s (normalised residual), x32 = 0
+// r32 = r64, x32 = 0 (residual replacement)
 __global__ __launch_bounds__(NT_UPD) void k_to32(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
-                                                 const double* __restrict__ scale, const double* __restrict__ r64,
-                                                 float* __restrict__ r32, float* __restrict__ x32) {
+                                                 const double* __restrict__ r64, float* __restrict__ r32,
+                                                 float* __restrict__ x32) {
     const BrickInfo B = bricks[blockIdx.x];
     if (done[B.dom]) return;
-    const int tid = threadIdx.x, lc = tid % R;
-    const double s = scale[B.dom * R + lc];
-    const double inv = s > 0.0 ? 1.0 / s : 0.0;
     const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
-    for (int64_t i = tid; i < cnt; i += NT_UPD) {
-        r32[base + i] = (float)(r64[base + i] * inv);
+    for (int64_t i = threadIdx.x; i < cnt; i += NT_UPD) {
+        r32[base + i] = (float)r64[base + i];
         x32[base + i] = 0.f;
     }
 }
 
-// x64 += s * x32
+// x64 += x32, x32 = 0 (group update of the FP32 correction)
 __global__ __launch_bounds__(NT_UPD) void k_acc64(const BrickInfo* __restrict__ bricks, const int* __restrict__ skip,
-                                                  const double* __restrict__ scale, const float* __restrict__ x32,
-                                                  double* __restrict__ x64) {
+                                                  float* __restrict__ x32, double* __restrict__ x64) {
     const BrickInfo B = bricks[blockIdx.x];
     if (skip[B.dom]) return;
-    const int tid = threadIdx.x, lc = tid % R;
-    const double s = scale[B.dom * R + lc];
-    if (s == 0.0) return;   // uniform per thread; no barrier below
     const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
-    for (int64_t i = tid; i < cnt; i += NT_UPD) x64[base + i] += s * (double)x32[base + i];
+    for (int64_t i = threadIdx.x; i < cnt; i += NT_UPD) {
+        x64[base + i] += (double)x32[base + i];
+        x32[base + i] = 0.f;
+    }
 }
 
 __global__ void k_fill_f64(double* __restrict__ a, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_f32(float* __restrict__ a, int64_t n, float v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
 }
@@ -1004,6 +1113,7 @@ int batch_finalize(SolverBatch& b, int precision) {
     for (DevMem* m : {&b.alpha, &b.beta, &b.rz, &b.rr, &b.rr0, &b.thr2, &b.scale}) DTC_TRY(m->alloc(nd * R * 8));
     DTC_TRY(b.active.ensure(nd * R * 4));
     DTC_TRY(b.active64.ensure(nd * R * 4));
+    DTC_TRY(b.active_prev.ensure(nd * R * 4));
     DTC_TRY(b.done.ensure(nd * 4));
     DTC_TRY(b.done64.ensure(nd * 4));
     DTC_TRY(b.iters.ensure(nd * 8));
@@ -1066,7 +1176,7 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
         DTC_TRY(event_pair(b, &e0, &e1));
         DTC_HIP(hipEventRecord(e0, b.stream));
     }
-    k_matvec<T, IN, OUT><<<b.n_bricks, MvCfg<T>::NT, 0, b.stream>>>(a, wk, xin, yout);
+    k_matvec<T, IN, OUT><<<b.n_bricks, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
     DTC_HIP(hipGetLastError());
     if (t) DTC_HIP(hipEventRecord(e1, b.stream));
     return DTC_OK;
@@ -1078,11 +1188,11 @@ int pcg_iteration(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const
     const int D = b.n_dom, NBk = b.n_bricks;
     int* done = b.done.as<int>();
     DTC_TRY((launch_mv<T, IN_V, OUT_MASKED_DOT>(b, wk, p, q, b.part_a.as<double>(), done)));
-    k_red_alpha<<<D, 64, 0, b.stream>>>(b.dom.as<DomInfo>(), done, b.part_a.as<double>(), b.active.as<int>(),
+    k_red_alpha<<<D, RED_NT, 0, b.stream>>>(b.dom.as<DomInfo>(), done, b.part_a.as<double>(), b.active.as<int>(),
                                         b.rz.as<double>(), b.alpha.as<double>());
     k_cg_update<T><<<NBk, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), done, b.alpha.as<double>(), x, r, p, q, dinv,
                                                  b.part_a.as<double>(), b.part_b.as<double>());
-    k_red_beta<<<D, 64, 0, b.stream>>>(b.dom.as<DomInfo>(), done, b.part_a.as<double>(), b.part_b.as<double>(),
+    k_red_beta<<<D, RED_NT, 0, b.stream>>>(b.dom.as<DomInfo>(), done, b.part_a.as<double>(), b.part_b.as<double>(),
                                        b.thr2.as<double>(), b.active.as<int>(), b.rz.as<double>(), b.beta.as<double>(),
                                        b.iters.as<int64_t>(), max_iters);
     k_cg_dir<T><<<NBk, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), done, b.beta.as<double>(), r, p, dinv);
@@ -1102,24 +1212,48 @@ int all_done(SolverBatch& b, const DevMem& flags, bool* out) {
 }
 
 template <typename T>
-int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* dinv, int64_t max_iters) {
+int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* dinv, int64_t max_iters,
+            int64_t max_steps) {
     constexpr int CHECK = 20;
-    for (;;) {
+    for (int64_t steps = 0; steps < max_steps; steps += CHECK) {
         bool fin = false;
         DTC_TRY(all_done(b, b.done, &fin));
         if (fin) return DTC_OK;
         for (int k = 0; k < CHECK; ++k) DTC_TRY(pcg_iteration<T>(b, wk, x, r, p, q, dinv, max_iters));
     }
+    return DTC_OK;
 }
 
-// FP64 true residual r = -(K u)_free of u = x64 + x_bc (x64 == null: u = x_bc)
+// FP64 true residual r64 = -(K u)_free, u = x64 + x_bc (x64 == null: u = x_bc),
+// then per load case activity and targets (k_red_outer)
 int residual64(SolverBatch& b, const WK<double>& wk, const double* x64, int first, const dtc_solve_opts& o) {
     DTC_TRY((launch_mv<double, IN_U, OUT_RESID>(b, wk, x64, b.r64.as<double>(), b.part_c.as<double>(), nullptr)));
-    k_red_outer<<<b.n_dom, 64, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_c.as<double>(), first, o.rel_tol * o.rel_tol,
-                                              o.inner_tol * o.inner_tol, b.precision, b.rr0.as<double>(),
-                                              b.rr.as<double>(), b.active64.as<int>(), b.active.as<int>(),
-                                              b.scale.as<double>(), b.thr2.as<double>(), b.done64.as<int>());
+    k_red_outer<<<b.n_dom, RED_NT, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_c.as<double>(), first,
+                                                  o.rel_tol * o.rel_tol, b.rr0.as<double>(), b.rr.as<double>(),
+                                                  b.active64.as<int>(), b.active.as<int>(), b.thr2.as<double>(),
+                                                  b.done64.as<int>(), b.done.as<int>());
     DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+// inner iteration cap reached for some domains: mark them finished
+int apply_cap(SolverBatch& b, int64_t cap, std::vector<int64_t>& nouter, bool* any_left) {
+    const int D = b.n_dom;
+    std::vector<int> h(D);
+    std::vector<int64_t> hi(D);
+    DTC_HIP(hipMemcpyAsync(h.data(), b.done64.p, (size_t)D * 4, hipMemcpyDeviceToHost, b.stream));
+    DTC_HIP(hipMemcpyAsync(hi.data(), b.iters.p, (size_t)D * 8, hipMemcpyDeviceToHost, b.stream));
+    DTC_HIP(hipStreamSynchronize(b.stream));
+    bool any = false, changed = false;
+    for (int d = 0; d < D; ++d) {
+        if (!h[d] && hi[d] >= cap) { h[d] = 1; changed = true; }
+        if (!h[d]) { nouter[d] += 1; any = true; }
+    }
+    if (changed) {
+        DTC_HIP(hipMemcpyAsync(b.done64.p, h.data(), (size_t)D * 4, hipMemcpyHostToDevice, b.stream));
+        DTC_HIP(hipMemcpyAsync(b.done.p, h.data(), (size_t)D * 4, hipMemcpyHostToDevice, b.stream));
+    }
+    *any_left = any;
     return DTC_OK;
 }
 
@@ -1137,58 +1271,72 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
     const WK<double> wk64 = cast_wk<double>(b.wk64);
     const WK<float> wk32 = cast_wk<float>(b.wk64);
     const int64_t cap = o.max_iter > 0 ? o.max_iter : 100000;
+    const BrickInfo* bl = b.brick.as<BrickInfo>();
+    const DomInfo* dl = b.dom.as<DomInfo>();
     k_fill_i64<<<nb(D, 64), 64, 0, s>>>(b.iters.as<int64_t>(), D, 0);
     k_fill_f64<<<nb(nv, 256), 256, 0, s>>>(b.x64.as<double>(), nv, 0.0);
+    if (b.precision == DTC_FP64) {
+        k_fill_f64<<<nb(nv, 256), 256, 0, s>>>(b.p64.as<double>(), nv, 0.0);
+        k_fill_f64<<<nb(nv, 256), 256, 0, s>>>(b.q64.as<double>(), nv, 0.0);
+    } else {
+        k_fill_f32<<<nb(nv, 256), 256, 0, s>>>(b.p32.as<float>(), nv, 0.f);
+        k_fill_f32<<<nb(nv, 256), 256, 0, s>>>(b.q32.as<float>(), nv, 0.f);
+    }
     DTC_HIP(hipGetLastError());
     b.mv_ms_sum = 0.0;
     b.mv_count = 0;
-    DTC_TRY(residual64(b, wk64, nullptr, 1, o));
+    b.reactions_valid = false;
     std::vector<int64_t> nouter(D, 0);
-    for (int it = 0; it < 64; ++it) {
-        bool fin = false;
-        DTC_TRY(all_done(b, b.done64, &fin));
-        if (fin) break;
-        {   // count outer rounds per domain and stop domains past the iteration cap
-            std::vector<int> h(D);
-            std::vector<int64_t> hi(D);
-            DTC_HIP(hipMemcpyAsync(h.data(), b.done64.p, (size_t)D * 4, hipMemcpyDeviceToHost, s));
-            DTC_HIP(hipMemcpyAsync(hi.data(), b.iters.p, (size_t)D * 8, hipMemcpyDeviceToHost, s));
-            DTC_HIP(hipStreamSynchronize(s));
-            bool any = false;
-            for (int d = 0; d < D; ++d) {
-                if (!h[d] && hi[d] >= cap) { h[d] = 1; }
-                if (!h[d]) { nouter[d] += 1; any = true; }
-            }
-            if (!any) break;
-            DTC_HIP(hipMemcpyAsync(b.done64.p, h.data(), (size_t)D * 4, hipMemcpyHostToDevice, s));
-            DTC_HIP(hipMemcpyAsync(b.done.p, h.data(), (size_t)D * 4, hipMemcpyHostToDevice, s));
-        }
-        if (b.precision == DTC_FP64) {
+    DTC_TRY(residual64(b, wk64, nullptr, 1, o));          // r64 = f (lifted right-hand side)
+
+    if (b.precision == DTC_FP64) {
+        // PCG in FP64 on (x64, r64); a true-residual check restarts it if the
+        // recursive residual drifted
+        for (int it = 0; it < 16; ++it) {
+            bool left = false;
+            DTC_TRY(apply_cap(b, cap, nouter, &left));
+            if (!left) break;
             double* x = b.x64.as<double>();
             double* r = b.r64.as<double>();
-            k_cg_init<double><<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), b.done64.as<int>(), b.active.as<int>(), x, r,
-                                                            b.p64.as<double>(), b.dinv64.as<double>(), b.part_a.as<double>(),
-                                                            b.part_b.as<double>(), 0);
-            k_red_init<<<D, 64, 0, s>>>(b.dom.as<DomInfo>(), b.part_a.as<double>(), b.part_b.as<double>(), b.thr2.as<double>(),
-                                        b.active.as<int>(), b.rz.as<double>(), b.done.as<int>());
+            k_cg_init<double><<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.active.as<int>(), x, r,
+                                                            b.p64.as<double>(), b.dinv64.as<double>(),
+                                                            b.part_a.as<double>(), b.part_b.as<double>(), 0, 1, nullptr);
+            k_red_init<<<D, RED_NT, 0, s>>>(dl, b.done64.as<int>(), b.part_a.as<double>(), b.part_b.as<double>(),
+                                            b.thr2.as<double>(), b.active.as<int>(), b.rz.as<double>(), b.done.as<int>());
             DTC_HIP(hipGetLastError());
-            DTC_TRY(pcg_run<double>(b, wk64, x, r, b.p64.as<double>(), b.q64.as<double>(), b.dinv64.as<double>(), cap));
-        } else {
-            k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), b.done64.as<int>(), b.scale.as<double>(),
-                                                 b.r64.as<double>(), b.r32.as<float>(), b.x32.as<float>());
-            k_cg_init<float><<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), b.done64.as<int>(), b.active.as<int>(),
-                                                           b.x32.as<float>(), b.r32.as<float>(), b.p32.as<float>(),
-                                                           b.dinv32.as<float>(), b.part_a.as<double>(), b.part_b.as<double>(), 0);
-            k_red_init<<<D, 64, 0, s>>>(b.dom.as<DomInfo>(), b.part_a.as<double>(), b.part_b.as<double>(), b.thr2.as<double>(),
-                                        b.active.as<int>(), b.rz.as<double>(), b.done.as<int>());
-            DTC_HIP(hipGetLastError());
-            DTC_TRY(pcg_run<float>(b, wk32, b.x32.as<float>(), b.r32.as<float>(), b.p32.as<float>(), b.q32.as<float>(),
-                                   b.dinv32.as<float>(), cap));
-            k_acc64<<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), b.done64.as<int>(), b.scale.as<double>(),
-                                                  b.x32.as<float>(), b.x64.as<double>());
-            DTC_HIP(hipGetLastError());
+            DTC_TRY(pcg_run<double>(b, wk64, x, r, b.p64.as<double>(), b.q64.as<double>(), b.dinv64.as<double>(), cap,
+                                    INT64_MAX));
+            DTC_TRY(residual64(b, wk64, b.x64.as<double>(), 0, o));
         }
-        DTC_TRY(residual64(b, wk64, b.x64.as<double>(), 0, o));
+    } else {
+        // FP32 PCG with FP64 residual replacement: the FP32 correction x32 is
+        // folded into x64 every REPLACE iterations (and when the recursive
+        // residual meets its target), r32 is replaced by the true FP64
+        // residual and the search direction p32 is kept.
+        constexpr int64_t REPLACE = 500;
+        float* x32 = b.x32.as<float>();
+        float* r32 = b.r32.as<float>();
+        float* p32 = b.p32.as<float>();
+        bool first = true;
+        for (int it = 0; it < 100000; ++it) {
+            bool left = false;
+            DTC_TRY(apply_cap(b, cap, nouter, &left));
+            if (!left) break;
+            k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.r64.as<double>(), r32, x32);
+            k_cg_init<float><<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.active.as<int>(), x32, r32, p32,
+                                                           b.dinv32.as<float>(), b.part_a.as<double>(),
+                                                           b.part_b.as<double>(), 0, first ? 1 : 0,
+                                                           b.active_prev.as<int>());
+            k_red_init<<<D, RED_NT, 0, s>>>(dl, b.done64.as<int>(), b.part_a.as<double>(), b.part_b.as<double>(),
+                                            b.thr2.as<double>(), b.active.as<int>(), b.rz.as<double>(), b.done.as<int>());
+            DTC_HIP(hipGetLastError());
+            first = false;
+            DTC_TRY(pcg_run<float>(b, wk32, x32, r32, p32, b.q32.as<float>(), b.dinv32.as<float>(), cap, REPLACE));
+            DTC_HIP(hipMemcpyAsync(b.active_prev.p, b.active.p, (size_t)D * R * 4, hipMemcpyDeviceToDevice, s));
+            k_acc64<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), x32, b.x64.as<double>());
+            DTC_HIP(hipGetLastError());
+            DTC_TRY(residual64(b, wk64, b.x64.as<double>(), 0, o));
+        }
     }
     // results: iterations, true relative residual, convergence status
     std::vector<int64_t> hi(D);
@@ -1197,6 +1345,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
     DTC_HIP(hipMemcpyAsync(hrr.data(), b.rr.p, (size_t)D * R * 8, hipMemcpyDeviceToHost, s));
     DTC_HIP(hipMemcpyAsync(hr0.data(), b.rr0.p, (size_t)D * R * 8, hipMemcpyDeviceToHost, s));
     DTC_HIP(hipStreamSynchronize(s));
+    DTC_TRY(harvest_events(b));
     for (int d = 0; d < D; ++d) {
         iters[d] = hi[d];
         outer[d] = nouter[d];

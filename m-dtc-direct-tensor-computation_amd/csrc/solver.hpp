@@ -28,6 +28,7 @@ struct SolverBatch {
     // per-domain scalars [dom][24]
     DevMem alpha, beta, rz, rr, rr0, thr2, scale;
     DevMem active, active64;       // int [dom][24]
+    DevMem active_prev;            // inner activity when the residual was last replaced
     DevMem done;                   // int [dom]: inner iteration finished
     DevMem done64;                 // int [dom]: outer (FP64 residual) converged
     DevMem iters;                  // int64 [dom]
