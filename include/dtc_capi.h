@@ -41,7 +41,11 @@ extern "C" {
 
 /* precision modes of the solve */
 #define DTC_FP64        0   /* Jacobi-PCG in FP64 throughout */
-#define DTC_FP32_REFINE 1   /* FP32 Jacobi-PCG inner solves wrapped in FP64 iterative refinement */
+#define DTC_FP32_REFINE 1   /* FP32 PCG inner solves wrapped in FP64 iterative refinement */
+
+/* preconditioners of the PCG (DTC_FP64 always uses Jacobi, as PETSc's PCJACOBI) */
+#define DTC_PC_JACOBI   0   /* diagonal scaling (the reference's choice) */
+#define DTC_PC_MG       1   /* geometric multigrid V-cycle over 2x2x2 voxel coarsening, Chebyshev-Jacobi smoothing */
 
 const char* dtc_last_error(void);
 int dtc_version(void);
@@ -110,6 +114,8 @@ typedef struct {
     int64_t max_iter;       /* cap on the total PCG iterations */
     double rot_step_deg;    /* initial grid step of the rotation search (1 = reference) */
     double num_zero;        /* refinement stop |dmin| < num_zero (1e-9 in mod_tensor_opt.f90:61) */
+    int precond;            /* DTC_PC_JACOBI or DTC_PC_MG (FP32_REFINE only) */
+    int mg_degree;          /* Chebyshev degree per smoothing pass (0 = default 2) */
 } dtc_solve_opts;
 
 typedef struct {

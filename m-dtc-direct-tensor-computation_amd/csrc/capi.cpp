@@ -435,6 +435,9 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         DTC_TRY(batch_add_domain(b->sb, b->mesh, d, b->mod[i]->p ? b->mod[i]->as<double>() : nullptr, b->escale[i], s));
     }
     DTC_TRY(batch_finalize(b->sb, b->o.precision));
+    b->sb.precond = (b->o.precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG) ? DTC_PC_MG : DTC_PC_JACOBI;
+    b->sb.mg_degree = b->o.mg_degree > 0 ? b->o.mg_degree : 2;
+    if (b->sb.precond == DTC_PC_MG) DTC_TRY(batch_build_mg(b->sb, 8));
     DTC_TRY(batch_solve(b->sb, b->o, b->iters, b->outer, b->relres));
     DTC_TRY(batch_forces(b->sb, b->ff));
     b->EE.assign((size_t)D * 36, 0.0);

@@ -39,6 +39,15 @@ struct DevMem {
     DevMem() = default;
     DevMem(const DevMem&) = delete;
     DevMem& operator=(const DevMem&) = delete;
+    DevMem(DevMem&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevMem& operator=(DevMem&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p; bytes = o.bytes;
+            o.p = nullptr; o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevMem() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -129,7 +138,12 @@ struct BrickInfo {
     int dom;
     int x0, y0, z0;           // first owned lattice node
     int node_begin, node_end; // owned solver nodes (contiguous)
+    int e_off, e_cnt;         // solid elements of the element tile in the brick element tables
 };
+
+// brick halo table entries: solver node | BC_BIT for a Dirichlet node, -1 = no node
+constexpr int BC_BIT = 1 << 30;
+constexpr int NODE_MASK = BC_BIT - 1;
 
 // 45 coefficients of the HEX08 element stiffness in the signed Walsh basis
 // (E = 1); see walsh_blocks() in solver.hip.

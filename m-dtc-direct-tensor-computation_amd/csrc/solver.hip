@@ -25,8 +25,11 @@
 //  * Precision: DTC_FP64 runs PCG in FP64; DTC_FP32_REFINE runs the inner
 //    PCG in FP32 on the residual equation and keeps x, the true residual
 //    r = -(K u)_free and the reactions in FP64.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "bc.hpp"
 #include "solver.hpp"
@@ -39,7 +42,7 @@ constexpr int R = R24;
 constexpr int NT_UPD = 192;    // multiple of 24: a thread keeps one load case
 
 enum InMode { IN_V = 0, IN_U = 1 };
-enum OutMode { OUT_MASKED_DOT = 0, OUT_RESID = 1, OUT_PLAIN = 2 };
+enum OutMode { OUT_MASKED_DOT = 0, OUT_RESID = 1, OUT_PLAIN = 2, OUT_MASKED = 3, OUT_CHEB = 4, OUT_BRES = 5 };
 
 template <typename T> struct WK { T g[45]; };
 
@@ -119,14 +122,22 @@ __device__ inline void walsh_apply(const WK<T>& k, const T w[8][3], T y[8][3]) {
 }
 
 // ---------------------------------------------------------------- matvec
-// Load cases are processed in chunks of RC; one (node, direction) segment of
-// a chunk is 32 bytes = two 16-byte vectors (float4 / double2).
+// Load cases are processed in chunks of RC: one (node, direction) segment of a
+// chunk is one 16-byte vector V (float4 / double2).
 template <typename T> struct MvCfg;
-template <> struct MvCfg<float> { static constexpr int RC = 8; using V = float4; };
-template <> struct MvCfg<double> { static constexpr int RC = 4; using V = double2; };
+template <> struct MvCfg<float> { static constexpr int RC = 4; using V = float4; };
+template <> struct MvCfg<double> { static constexpr int RC = 2; using V = double2; };
 constexpr int MV_NT = 256;
-constexpr int VPS = 2;          // 16-byte vectors per (node, direction) segment
-constexpr int VPN = 3 * VPS;    // per node and chunk
+constexpr int NH = HX * HY * HZ, NET = EX * EY * EZ, NO = BX * BY * BZ;
+constexpr int OB_BIAS = 1 + BX + BX * BY;   // owned-layout base of tile element (0,0,0) is -OB_BIAS
+
+// offsets of element corner b (bit 0 x, bit 1 y, bit 2 z) in the halo and owned node layouts
+__host__ __device__ constexpr int hcorner(int b) { return (b & 1) + HX * ((b >> 1) & 1) + HX * HY * (b >> 2); }
+__host__ __device__ constexpr int ocorner(int b) { return (b & 1) + BX * ((b >> 1) & 1) + BX * BY * (b >> 2); }
+// halo index of owned node o
+__host__ __device__ constexpr int ohalo(int o) {
+    return (o % BX + 1) + HX * ((o / BX) % BY + 1) + HX * HY * (o / (BX * BY) + 1);
+}
 
 __device__ inline float vcomp(const float4& v, int w) { return w == 0 ? v.x : w == 1 ? v.y : w == 2 ? v.z : v.w; }
 __device__ inline double vcomp(const double2& v, int w) { return w == 0 ? v.x : v.y; }
@@ -134,214 +145,258 @@ __device__ inline void vzero(float4& v) { v = make_float4(0.f, 0.f, 0.f, 0.f); }
 __device__ inline void vzero(double2& v) { v = make_double2(0.0, 0.0); }
 __device__ inline float4 vneg(const float4& v) { return make_float4(-v.x, -v.y, -v.z, -v.w); }
 __device__ inline double2 vneg(const double2& v) { return make_double2(-v.x, -v.y); }
+__device__ inline void vadd(float4& a, const float4& b) { a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w; }
+__device__ inline void vadd(double2& a, const double2& b) { a.x += b.x; a.y += b.y; }
 
 struct MvArgs {
     const BrickInfo* bricks;
     const DomInfo* doms;
-    const int* lat2node;
-    const double* emod;
-    const int* done;       // may be null
-    double* partial;       // [brick][24]
+    const int* halo;            // [brick][NH]
+    const unsigned char* nmask; // [brick][NO]: solid tile elements around each owned node
+    const int* einfo;           // tile elements, packed
+    const void* eE;             // tile element moduli (T)
+    const int* done;            // may be null
+    double* partial;            // [brick][24]
+    // OUT_CHEB / OUT_BRES only
+    const void* bvec;           // right-hand side b
+    const void* dinv;           // Jacobi scaling
+    void* dvec;                 // Chebyshev direction (read and written)
+    float c1, c2;               // Chebyshev coefficients of this step
 };
 
-// Parity classes of the element tile: elements (ex,ey,ez) with the same
-// parities share no node, so one class accumulates into LDS without conflicts.
-// element products of one class (one task = solid element x load case)
-template <typename T>
-__device__ __forceinline__ void mv_class(int ncls, const short* __restrict__ elist, const WK<T>& wk,
-                                         const T* __restrict__ s_E, const T* __restrict__ s_x, T* __restrict__ s_y,
-                                         int tid) {
-    constexpr int RC = MvCfg<T>::RC, SX = 3 * RC;
-    const int ntask = ncls * RC;
-#pragma unroll 1
-    for (int t = tid; t < ntask; t += MV_NT) {
-        const int r = t % RC;
-        const int i = elist[t / RC];
-        const int ex = i % EX, ey = (i / EX) % EY, ez = i / (EX * EY);
-        const T E = s_E[i];
-        T w[8][3], y[8][3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            T v[8];
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int h = (ex + (b & 1)) + HX * ((ey + ((b >> 1) & 1)) + HY * (ez + (b >> 2)));
-                v[b] = s_x[h * SX + d * RC + r];
-            }
-            fht_fwd(v);
-#pragma unroll
-            for (int b = 0; b < 8; ++b) w[b][d] = v[b];
-        }
-        walsh_apply(wk, w, y);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            T v[8];
-#pragma unroll
-            for (int b = 0; b < 8; ++b) v[b] = y[b][d] * E;
-            fht_inv(v);
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int hx = ex + (b & 1), hy = ey + ((b >> 1) & 1), hz = ez + (b >> 2);
-                if (hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ) {
-                    const int o = (hx - 1) + BX * ((hy - 1) + BY * (hz - 1));
-                    s_y[o * SX + d * RC + r] += v[b];
-                }
-            }
-        }
-    }
-}
-
+// y = K x for every brick of the batch.  One workgroup per brick and chunk
+// loop over the load cases:
+//   stage   the halo of x (10x6x6 nodes) goes to LDS (16-byte loads, issued
+//           one chunk ahead into registers);
+//   product every solid element of the 9x5x5 tile is one task per load case
+//           (Walsh-basis product); the output of corner b lands in the corner
+//           buffer b at the owned node it belongs to.  A (corner, node) slot
+//           has exactly one element, so plain stores suffice: no atomics, no
+//           colouring barriers;
+//   gather  each owned (node, direction) sums its corner buffers in corner
+//           order (deterministic) and is written once.
+// OUT_MASKED_DOT zeroes Dirichlet rows and writes per-brick sums of x.y;
+// OUT_RESID writes -y on free rows and sums y.y; OUT_PLAIN writes y;
+// OUT_MASKED zeroes Dirichlet rows; OUT_BRES writes b - y on free rows and 0 on
+// Dirichlet rows; OUT_CHEB applies one Chebyshev-Jacobi step to the owned nodes,
+//   d = c1 d + c2 D^-1 (b - y),  yout = x + d   (yout must not alias x),
+// and, when partial is set, writes per-brick sums of b . yout.
 template <typename T, int IN, int OUT>
-__global__ __launch_bounds__(MV_NT, 3) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
+__global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
                                                  T* __restrict__ yout) {
-    constexpr int RC = MvCfg<T>::RC, SX = 3 * RC, VW = 16 / (int)sizeof(T);
+    constexpr int RC = MvCfg<T>::RC, SX = 3 * RC, NCH = R / RC;
     using V = typename MvCfg<T>::V;
-    constexpr int NH = HX * HY * HZ, NE = EX * EY * EZ, NO = BX * BY * BZ;
-    constexpr int NLD = (NH * VPN + MV_NT - 1) / MV_NT;
-    constexpr int NOP = MV_NT / VPN;              // owned nodes per pass of the epilogue
+    constexpr int NLD = (NH * 3 + MV_NT - 1) / MV_NT;
+    constexpr int NW = MV_NT / 64;
     __shared__ int s_node[NH];
-    __shared__ unsigned char s_bc[NH];
-    __shared__ T s_E[NE];
-    constexpr int MAXC = ((EX + 1) / 2) * ((EY + 1) / 2) * ((EZ + 1) / 2);   // largest parity class
-    __shared__ short s_elist[8 * MAXC];           // solid elements of the tile, per parity class
-    __shared__ int s_ccount[8];
-    __shared__ __align__(16) T s_x[NH * SX];      // also the reduction scratch after the epilogue
-    __shared__ __align__(16) T s_y[NO * SX];
-    V* s_xv = reinterpret_cast<V*>(s_x);
-    V* s_yv = reinterpret_cast<V*>(s_y);
-    double* s_red = reinterpret_cast<double*>(s_x);
+    __shared__ int s_info[NET];
+    __shared__ T s_E[NET];
+    __shared__ unsigned char s_nm[NO];
+    __shared__ double s_wred[NCH][NW][RC];
+    __shared__ V s_x[NH * 3];
+    __shared__ V s_c[(OB_BIAS + 8 * NO) * 3];   // corner buffers; the bias keeps element bases non-negative
 
     const BrickInfo B = A.bricks[blockIdx.x];
     if (A.done && A.done[B.dom]) return;
-    const DomInfo D = A.doms[B.dom];
     const int tid = threadIdx.x;
-    const int Lx = D.nx + 1, Ly = D.ny + 1, Lz = D.nz + 1;
-
-    for (int i = tid; i < NH; i += MV_NT) {
-        const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
-        const int gx = B.x0 - 1 + hx, gy = B.y0 - 1 + hy, gz = B.z0 - 1 + hz;
-        int q = -1;
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < Lx && gy < Ly && gz < Lz)
-            q = A.lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
-        s_node[i] = q;
-        s_bc[i] = (q >= 0 && is_bc(D, gx, gy, gz)) ? 1 : 0;
-    }
-    if (tid < 8) s_ccount[tid] = 0;
-    __syncthreads();
-    for (int i = tid; i < NE; i += MV_NT) {
-        const int ex = i % EX, ey = (i / EX) % EY, ez = i / (EX * EY);
-        const int vx = B.x0 - 1 + ex, vy = B.y0 - 1 + ey, vz = B.z0 - 1 + ez;
-        T e = T(0);
-        if (vx >= 0 && vy >= 0 && vz >= 0 && vx < D.nx && vy < D.ny && vz < D.nz)
-            e = (T)A.emod[D.vox_off + vx + (int64_t)D.nx * (vy + (int64_t)D.ny * vz)];
-        s_E[i] = e;
-        if (e != T(0)) {
-            // order inside a class does not matter: a class adds at most once to a node
-            const int c = (ex & 1) | ((ey & 1) << 1) | ((ez & 1) << 2);
-            s_elist[c * MAXC + atomicAdd(&s_ccount[c], 1)] = (short)i;
-        }
-    }
-    __syncthreads();
-
+    const int* __restrict__ halo = A.halo + (int64_t)blockIdx.x * NH;
     const V* xv = reinterpret_cast<const V*>(xin);
     V* yv = reinterpret_cast<V*>(yout);
-    V reg[NLD];
-    // 16-byte loads of one chunk into registers (IN_V); issued before the element
-    // phase of the previous chunk so their latency overlaps the compute
-    auto prefetch = [&](int chunk) {
+
+    // x loads run two chunks ahead: chunk c is staged from regs[c % 2]
+    static_assert(NCH % 2 == 0, "chunk loop is unrolled by two");
+    V regA[NLD], regB[NLD];
+    auto prefetch = [&](V (&reg)[NLD], int chunk, const int* nodes) {
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
             const int i = tid + k * MV_NT;
             V v;
             vzero(v);
-            if (i < NH * VPN) {
-                const int h = i / VPN, j = i % VPN, d = j / VPS, c = j % VPS;
-                const int q = s_node[h];
-                if (q >= 0) v = xv[(((int64_t)q * 3 + d) * R + chunk * RC) / VW + c];
+            if (i < NH * 3) {
+                const int q = nodes[i / 3];
+                if (q >= 0) v = xv[((int64_t)(q & NODE_MASK) * 3 + i % 3) * NCH + chunk];
             }
             reg[k] = v;
         }
     };
-    if (IN == IN_V) prefetch(0);
+    if (IN == IN_V) {
+        prefetch(regA, 0, halo);
+        prefetch(regB, 1, halo);
+    }
+    for (int i = tid; i < NH; i += MV_NT) s_node[i] = halo[i];
+    if (tid < NO) s_nm[tid] = A.nmask[(int64_t)blockIdx.x * NO + tid];
+    const int ne = B.e_cnt;
+    {
+        const T* __restrict__ eE = static_cast<const T*>(A.eE) + B.e_off;
+        const int* __restrict__ ei = A.einfo + B.e_off;
+        for (int i = tid; i < ne; i += MV_NT) {
+            s_info[i] = ei[i];
+            s_E[i] = eE[i];
+        }
+    }
+    if (IN != IN_V) __syncthreads();
 
-    for (int chunk = 0; chunk < R / RC; ++chunk) {
+    auto chunk_body = [&](int chunk, V (&reg)[NLD]) {
         if (IN == IN_V) {
 #pragma unroll
             for (int k = 0; k < NLD; ++k) {
                 const int i = tid + k * MV_NT;
-                if (i < NH * VPN) s_xv[i] = reg[k];
+                if (i < NH * 3) s_x[i] = reg[k];
             }
         } else {
+            const DomInfo D = A.doms[B.dom];
+            T* sx = reinterpret_cast<T*>(s_x);
             for (int i = tid; i < NH * SX; i += MV_NT) {
                 const int h = i / SX, rem = i % SX, d = rem / RC, r = rem % RC;
                 const int lc = chunk * RC + r;
                 const int q = s_node[h];
                 T v = T(0);
                 if (q >= 0) {
-                    if (s_bc[h]) {
+                    if (q & BC_BIT) {
                         const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
                         double c[3], N[8];
                         node_coord(D, B.x0 - 1 + hx, B.y0 - 1 + hy, B.z0 - 1 + hz, c);
                         macro_shape8(c, D.min_c, D.max_c, N);
                         v = (T)bc_value(N, d, lc);
                     } else if (xin) {
-                        v = xin[((int64_t)q * 3 + d) * R + lc];
+                        v = xin[((int64_t)(q & NODE_MASK) * 3 + d) * R + lc];
                     }
                 }
-                s_x[i] = v;
+                sx[i] = v;
             }
         }
-        for (int i = tid; i < NO * VPN; i += MV_NT) vzero(s_yv[i]);
         __syncthreads();
-        if (IN == IN_V && chunk + 1 < R / RC) prefetch(chunk + 1);
+        if (IN == IN_V && chunk + 2 < NCH) prefetch(reg, chunk + 2, s_node);
 
+        // ---- element products: task t = (tile element t / RC, load case t % RC)
+        {
+            const T* sx = reinterpret_cast<const T*>(s_x);
+            T* sc = reinterpret_cast<T*>(s_c);
+            const int ntask = ne * RC;
 #pragma unroll 1
-        for (int cls = 0; cls < 8; ++cls) {
-            const int ncls = s_ccount[cls];
-            if (ncls == 0) continue;     // uniform across the block
-            mv_class<T>(ncls, s_elist + cls * MAXC, wk, s_E, s_x, s_y, tid);
-            __syncthreads();
-        }
-
-        // ---- write owned nodes: thread keeps one 16-byte column jj of the node segment
-        const int jj = tid % VPN, d = jj / VPS, c = jj % VPS;
-        double acc[VW];
+            for (int t = tid; t < ntask; t += MV_NT) {
+                const int r = t % RC, k = t / RC;
+                const int info = s_info[k];
+                const T E = s_E[k];
+                const unsigned mask = (unsigned)info >> 24;
+                const T* xb = sx + (info & 511) * SX + r;
+                T* cb = sc + ((info >> 9) & 255) * SX + r;
+                T w[8][3], y[8][3];
 #pragma unroll
-        for (int w = 0; w < VW; ++w) acc[w] = 0.0;
-        if (tid < NOP * VPN) {
-            for (int o = tid / VPN; o < NO; o += NOP) {
-                const int h = (o % BX + 1) + HX * ((o / BX) % BY + 1) + HX * HY * (o / (BX * BY) + 1);
-                const int q = s_node[h];
-                if (q < 0) continue;
-                V y = s_yv[o * VPN + jj];
-                if (OUT == OUT_MASKED_DOT) {
-                    if (s_bc[h]) vzero(y);
-                    const V x = s_xv[h * VPN + jj];
+                for (int d = 0; d < 3; ++d) {
+                    T v[8];
 #pragma unroll
-                    for (int w = 0; w < VW; ++w) acc[w] += (double)vcomp(x, w) * (double)vcomp(y, w);
-                } else if (OUT == OUT_RESID) {
-                    if (s_bc[h]) vzero(y); else y = vneg(y);
+                    for (int b = 0; b < 8; ++b) v[b] = xb[hcorner(b) * SX + d * RC];
+                    fht_fwd(v);
 #pragma unroll
-                    for (int w = 0; w < VW; ++w) acc[w] += (double)vcomp(y, w) * (double)vcomp(y, w);
+                    for (int b = 0; b < 8; ++b) w[b][d] = v[b];
                 }
-                yv[(((int64_t)q * 3 + d) * R + chunk * RC) / VW + c] = y;
+                walsh_apply(wk, w, y);
+                T o[3][8];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) o[d][b] = y[b][d] * E;
+                    fht_inv(o[d]);
+                }
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    if (mask & (1u << b)) {
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) cb[(b * NO + ocorner(b)) * SX + d * RC] = o[d][b];
+                    }
             }
         }
         __syncthreads();
-        if (OUT != OUT_PLAIN) {
+
+        // ---- gather: thread per owned (node, direction), all RC load cases in one vector
+        double acc[RC];
 #pragma unroll
-            for (int w = 0; w < VW; ++w) s_red[tid * VW + w] = acc[w];
-            __syncthreads();
-            if (tid < RC) {
-                // load case r of the chunk lives in column c = r / VW, lane w = r % VW;
-                // threads with jj % VPS == c hold it (VPN is even, so tid % 2 == c)
-                const int cc = tid / VW, w = tid % VW;
-                double sum = 0.0;
-                for (int t = cc; t < NOP * VPN; t += VPS) sum += s_red[t * VW + w];
-                A.partial[(int64_t)blockIdx.x * R + chunk * RC + tid] = sum;
+        for (int w = 0; w < RC; ++w) acc[w] = 0.0;
+        for (int i = tid; i < NO * 3; i += MV_NT) {
+            const int o = i / 3, d = i % 3;
+            const int h = ohalo(o);
+            const int q = s_node[h];
+            if (q < 0) continue;
+            const unsigned m = s_nm[o];
+            V y;
+            vzero(y);
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+                if (m & (1u << b)) vadd(y, s_c[(OB_BIAS + b * NO + o) * 3 + d]);
+            const bool bc = (q & BC_BIT) != 0;
+            if (OUT == OUT_MASKED_DOT) {
+                if (bc) vzero(y);
+                const V x = s_x[h * 3 + d];
+#pragma unroll
+                for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(x, w) * (double)vcomp(y, w);
+            } else if (OUT == OUT_RESID) {
+                if (bc) vzero(y); else y = vneg(y);
+#pragma unroll
+                for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(y, w) * (double)vcomp(y, w);
+            } else if (OUT == OUT_MASKED) {
+                if (bc) vzero(y);
+            } else if (OUT == OUT_BRES || OUT == OUT_CHEB) {
+                const int64_t vi = ((int64_t)(q & NODE_MASK) * 3 + d) * NCH + chunk;
+                const V bv = reinterpret_cast<const V*>(A.bvec)[vi];
+                if (OUT == OUT_BRES) {
+                    if (bc) vzero(y);
+                    else {
+                        V t = bv;
+                        vadd(t, vneg(y));
+                        y = t;
+                    }
+                } else {
+                    V* dp = reinterpret_cast<V*>(A.dvec) + vi;
+                    V dn, xn;
+                    vzero(dn);
+                    vzero(xn);
+                    if (!bc) {
+                        const T di = reinterpret_cast<const T*>(A.dinv)[q & NODE_MASK];
+                        const V dv = *dp, xv = s_x[h * 3 + d];
+                        T* dnp = reinterpret_cast<T*>(&dn);
+                        T* xnp = reinterpret_cast<T*>(&xn);
+#pragma unroll
+                        for (int w = 0; w < RC; ++w) {
+                            const T r = vcomp(bv, w) - vcomp(y, w);
+                            dnp[w] = (T)A.c1 * vcomp(dv, w) + (T)A.c2 * di * r;
+                            xnp[w] = vcomp(xv, w) + dnp[w];
+                        }
+                    }
+                    *dp = dn;
+                    y = xn;
+                    if (A.partial)
+#pragma unroll
+                        for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(bv, w) * (double)vcomp(y, w);
+                }
             }
-            __syncthreads();
+            yv[((int64_t)(q & NODE_MASK) * 3 + d) * NCH + chunk] = y;
         }
+        if (OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) {
+            // every lane holds the RC load cases of the chunk: fixed butterfly over the wave
+#pragma unroll
+            for (int w = 0; w < RC; ++w) {
+                double v = acc[w];
+#pragma unroll
+                for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m);
+                acc[w] = v;
+            }
+            if (tid % 64 == 0)
+#pragma unroll
+                for (int w = 0; w < RC; ++w) s_wred[chunk][tid / 64][w] = acc[w];
+        }
+        __syncthreads();
+    };
+    for (int chunk = 0; chunk < NCH; chunk += 2) {
+        chunk_body(chunk, regA);
+        chunk_body(chunk + 1, regB);
+    }
+    if ((OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) && tid < R) {
+        const int ch = tid / RC, r = tid % RC;
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s += s_wred[ch][w][r];
+        A.partial[(int64_t)blockIdx.x * R + tid] = s;
     }
 }
 
@@ -353,7 +408,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_init(const BrickInfo* __restrict_
                                                     const T* __restrict__ r, T* __restrict__ p,
                                                     const T* __restrict__ dinv, double* __restrict__ prz,
                                                     double* __restrict__ prr, int zero_x, int set_p,
-                                                    const int* __restrict__ keep_p) {
+                                                    const int* __restrict__ keep_p, const T* __restrict__ zin) {
     __shared__ double s1[NT_UPD], s2[NT_UPD];
     const BrickInfo B = bricks[blockIdx.x];
     const int tid = threadIdx.x, lc = tid % R;
@@ -368,7 +423,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_init(const BrickInfo* __restrict_
             int64_t idx = base + i;
             int64_t node = idx / (3 * R);
             T rv = r[idx];
-            T z = act ? rv * dinv[node] : T(0);
+            T z = act ? (zin ? zin[idx] : rv * dinv[node]) : T(0);
             if (act && sp) p[idx] = z;
             if (zero_x) x[idx] = T(0);
             a1 += (double)rv * (double)z;
@@ -407,8 +462,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_update(const BrickInfo* __restric
             x[idx] += a * p[idx];
             r[idx] = rv;
         }
-        T z = rv * dinv[node];
-        a1 += (double)rv * (double)z;
+        if (prz) a1 += (double)rv * (double)(rv * dinv[node]);
         a2 += (double)rv * (double)rv;
     }
     s1[tid] = a1; s2[tid] = a2;
@@ -416,7 +470,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_update(const BrickInfo* __restric
     if (tid < R) {
         double u = 0.0, v = 0.0;
         for (int k = tid; k < NT_UPD; k += R) { u += s1[k]; v += s2[k]; }
-        prz[(int64_t)blockIdx.x * R + tid] = u;
+        if (prz) prz[(int64_t)blockIdx.x * R + tid] = u;
         prr[(int64_t)blockIdx.x * R + tid] = v;
     }
 }
@@ -424,7 +478,8 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_update(const BrickInfo* __restric
 template <typename T>
 __global__ __launch_bounds__(NT_UPD) void k_cg_dir(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
                                                    const double* __restrict__ beta, const T* __restrict__ r,
-                                                   T* __restrict__ p, const T* __restrict__ dinv) {
+                                                   T* __restrict__ p, const T* __restrict__ dinv,
+                                                   const T* __restrict__ zin) {
     const BrickInfo B = bricks[blockIdx.x];
     if (done && done[B.dom]) return;
     const int tid = threadIdx.x, lc = tid % R;
@@ -433,7 +488,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_dir(const BrickInfo* __restrict__
     for (int64_t i = tid; i < cnt; i += NT_UPD) {
         int64_t idx = base + i;
         int64_t node = idx / (3 * R);
-        p[idx] = r[idx] * dinv[node] + b * p[idx];
+        p[idx] = (zin ? zin[idx] : r[idx] * dinv[node]) + b * p[idx];
     }
 }
 
@@ -545,10 +600,12 @@ __global__ __launch_bounds__(RED_NT) void k_red_beta(const DomInfo* __restrict__
 }
 
 // outer (FP64) residual bookkeeping after a RESID pass: rr = ||f - K u||^2 per load case;
-// load cases above tol^2 ||f||^2 stay active; the recursive PCG residual
-// target is (tol/2)^2 ||f||^2 so the true residual check passes after drift
+// load cases above tol^2 ||f||^2 stay active.  The recursive PCG residual target
+// is the larger of (tol/2)^2 ||f||^2 (so the true residual check passes after
+// drift) and itol2 * rr (an inner solve reduces the residual it started from by
+// at most sqrt(itol2); itol2 = 0: no relative target)
 __global__ __launch_bounds__(RED_NT) void k_red_outer(const DomInfo* __restrict__ doms, const double* __restrict__ prr,
-                                                      int first, double tol2, double* __restrict__ rr0,
+                                                      int first, double tol2, double itol2, double* __restrict__ rr0,
                                                       double* __restrict__ rrcur, int* __restrict__ active64,
                                                       int* __restrict__ active, double* __restrict__ thr2,
                                                       int* __restrict__ done64, int* __restrict__ done) {
@@ -566,7 +623,7 @@ __global__ __launch_bounds__(RED_NT) void k_red_outer(const DomInfo* __restrict_
             const int a = (r0 > 0.0) && (rr[lc] > tol2 * r0);
             active64[d * R + lc] = a;
             active[d * R + lc] = a;
-            thr2[d * R + lc] = 0.25 * tol2 * r0;
+            thr2[d * R + lc] = fmax(0.25 * tol2 * r0, itol2 * rr[lc]);
             any |= a;
         }
         done64[d] = any ? 0 : 1;
@@ -707,6 +764,79 @@ __global__ void k_dinv(const BrickInfo* __restrict__ bricks, const DomInfo* __re
     if (is_bc(D, gx, gy, gz)) inv = 0.0;        // constrained DOFs stay zero
     d64[q] = inv;
     d32[q] = (float)inv;
+}
+
+// ---------------------------------------------------------------- matvec tables
+// solid elements of the element tile of each brick (blockDim >= NET)
+__global__ void k_brick_ecount(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
+                               const double* __restrict__ emod, int* __restrict__ cnt) {
+    __shared__ int s;
+    const BrickInfo B = bricks[blockIdx.x];
+    const DomInfo D = doms[B.dom];
+    const int i = threadIdx.x;
+    if (i == 0) s = 0;
+    __syncthreads();
+    if (i < NET) {
+        const int vx = B.x0 - 1 + i % EX, vy = B.y0 - 1 + (i / EX) % EY, vz = B.z0 - 1 + i / (EX * EY);
+        if (vx >= 0 && vy >= 0 && vz >= 0 && vx < D.nx && vy < D.ny && vz < D.nz &&
+            emod[D.vox_off + vx + (int64_t)D.nx * (vy + (int64_t)D.ny * vz)] != 0.0)
+            atomicAdd(&s, 1);
+    }
+    __syncthreads();
+    if (i == 0) cnt[blockIdx.x] = s;
+}
+
+// halo node table (solver node | BC_BIT) and the packed tile element list of
+// each brick, elements in lexicographic tile order (blockDim >= NET)
+__global__ void k_brick_tables(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
+                               const int* __restrict__ lat2node, const double* __restrict__ emod,
+                               int* __restrict__ halo, unsigned char* __restrict__ nmask, int* __restrict__ einfo,
+                               double* __restrict__ eE64, float* __restrict__ eE32) {
+    __shared__ int s_flag[NET];
+    const BrickInfo B = bricks[blockIdx.x];
+    const DomInfo D = doms[B.dom];
+    const int tid = threadIdx.x;
+    const int Lx = D.nx + 1, Ly = D.ny + 1, Lz = D.nz + 1;
+    for (int i = tid; i < NH; i += blockDim.x) {
+        const int gx = B.x0 - 1 + i % HX, gy = B.y0 - 1 + (i / HX) % HY, gz = B.z0 - 1 + i / (HX * HY);
+        int q = -1;
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < Lx && gy < Ly && gz < Lz) {
+            q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
+            if (q >= 0 && is_bc(D, gx, gy, gz)) q |= BC_BIT;
+        }
+        halo[(int64_t)blockIdx.x * NH + i] = q;
+    }
+    double e = 0.0;
+    const int ex = tid % EX, ey = (tid / EX) % EY, ez = tid / (EX * EY);
+    if (tid < NET) {
+        const int vx = B.x0 - 1 + ex, vy = B.y0 - 1 + ey, vz = B.z0 - 1 + ez;
+        if (vx >= 0 && vy >= 0 && vz >= 0 && vx < D.nx && vy < D.ny && vz < D.nz)
+            e = emod[D.vox_off + vx + (int64_t)D.nx * (vy + (int64_t)D.ny * vz)];
+        s_flag[tid] = e != 0.0 ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid < NO) {   // solid tile elements around owned node tid, bit b = element at corner offset -b
+        const int hx = tid % BX + 1, hy = (tid / BX) % BY + 1, hz = tid / (BX * BY) + 1;
+        unsigned m = 0;
+        for (int b = 0; b < 8; ++b)
+            if (s_flag[(hx - (b & 1)) + EX * ((hy - ((b >> 1) & 1)) + EY * (hz - (b >> 2)))]) m |= 1u << b;
+        nmask[(int64_t)blockIdx.x * NO + tid] = (unsigned char)m;
+    }
+    if (tid < NET && e != 0.0) {
+        int rank = 0;
+        for (int k = 0; k < tid; ++k) rank += s_flag[k];
+        unsigned mask = 0;
+        for (int b = 0; b < 8; ++b) {
+            const int hx = ex + (b & 1), hy = ey + ((b >> 1) & 1), hz = ez + (b >> 2);
+            if (hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ) mask |= 1u << b;
+        }
+        const int hb = ex + HX * (ey + HY * ez);
+        const int ob = (ex - 1) + BX * ((ey - 1) + BY * (ez - 1)) + OB_BIAS;
+        const int64_t o = (int64_t)B.e_off + rank;
+        einfo[o] = (int)((unsigned)hb | ((unsigned)ob << 9) | (mask << 24));
+        eE64[o] = e;
+        eE32[o] = (float)e;
+    }
 }
 
 // ---------------------------------------------------------------- force projection
@@ -871,6 +1001,209 @@ __global__ void k_grad_elem(const DomInfo* __restrict__ doms, const int* __restr
         }
     }
     g[v] = out;
+}
+
+// ---------------------------------------------------------------- multigrid (FP32 preconditioner)
+// Level l+1 has voxels of 2x2x2 voxels of level l.  Its voxel modulus is the sum
+// of the 8 children / 4: their mean times 2, because the element stiffness is
+// linear in the edge length and the Walsh table is the one of the fine edge.
+__global__ void k_coarsen(const double* __restrict__ ef, int fx, int fy, int fz, double* __restrict__ ec, int cx,
+                          int cy, int cz) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (int64_t)cx * cy * cz) return;
+    const int x = (int)(v % cx), y = (int)((v / cx) % cy), z = (int)(v / ((int64_t)cx * cy));
+    double s = 0.0;
+    for (int k = 0; k < 2; ++k)
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i) {
+                const int X = 2 * x + i, Y = 2 * y + j, Z = 2 * z + k;
+                if (X < fx && Y < fy && Z < fz) s += ef[X + (int64_t)fx * (Y + (int64_t)fy * Z)];
+            }
+    ec[v] = 0.25 * s;
+}
+
+// lattice node flag: 1 if a voxel touching the node carries a modulus
+__global__ void k_exist(const double* __restrict__ e, int nx, int ny, int nz, int* __restrict__ ex) {
+    const int Lx = nx + 1, Ly = ny + 1, Lz = nz + 1;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)Lx * Ly * Lz) return;
+    const int x = (int)(p % Lx), y = (int)((p / Lx) % Ly), z = (int)(p / ((int64_t)Lx * Ly));
+    int f = 0;
+    for (int dz = -1; dz <= 0; ++dz)
+        for (int dy = -1; dy <= 0; ++dy)
+            for (int dx = -1; dx <= 0; ++dx) {
+                const int vx = x + dx, vy = y + dy, vz = z + dz;
+                if (vx < 0 || vy < 0 || vz < 0 || vx >= nx || vy >= ny || vz >= nz) continue;
+                if (e[vx + (int64_t)nx * (vy + (int64_t)ny * vz)] != 0.0) f = 1;
+            }
+    ex[p] = f;
+}
+
+// nodes per brick from a node flag array (blockDim = BX*BY*BZ)
+__global__ void k_brick_count_ex(const int* __restrict__ ex, int Lx, int Ly, int Lz, int bnx, int bny,
+                                 int* __restrict__ cnt) {
+    __shared__ int s;
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (t == 0) s = 0;
+    __syncthreads();
+    const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
+    const int gx = bx * BX + t % BX, gy = by * BY + (t / BX) % BY, gz = bz * BZ + t / (BX * BY);
+    if (gx < Lx && gy < Ly && gz < Lz && ex[gx + Lx * (gy + Ly * gz)]) atomicAdd(&s, 1);
+    __syncthreads();
+    if (t == 0) cnt[b] = s;
+}
+
+// solver node numbers of the flagged nodes, lexicographic inside each brick
+__global__ void k_brick_assign_ex(const BrickInfo* __restrict__ bl, const int* __restrict__ ex, int Lx, int Ly, int Lz,
+                                  int64_t lat_off, int* __restrict__ lat2node) {
+    __shared__ int flag[NO];
+    const BrickInfo B = bl[blockIdx.x];
+    const int t = threadIdx.x;
+    const int gx = B.x0 + t % BX, gy = B.y0 + (t / BX) % BY, gz = B.z0 + t / (BX * BY);
+    int f = 0;
+    if (gx < Lx && gy < Ly && gz < Lz) f = ex[gx + Lx * (gy + Ly * gz)];
+    flag[t] = f;
+    __syncthreads();
+    if (f) {
+        int rank = 0;
+        for (int k = 0; k < t; ++k) rank += flag[k];
+        lat2node[lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)] = B.node_begin + rank;
+    }
+}
+
+// Transfer kernels: 18 lanes per node, each lane one float4 = 4 load cases of one
+// direction of the node's [3][24] segment.
+constexpr int T_LPN = 3 * R / 4;           // float4 per node
+constexpr int T_NPP = 256 / T_LPN;         // nodes per pass of a 256-thread workgroup
+
+__device__ inline void f4_axpy(float4& s, float w, const float4& a) {
+    s.x += w * a.x; s.y += w * a.y; s.z += w * a.z; s.w += w * a.w;
+}
+
+// b_c = P^T r_f: trilinear weights over the free (interior) fine nodes; one
+// workgroup per coarse brick
+__global__ __launch_bounds__(256) void k_restrict(const BrickInfo* __restrict__ cbr, const DomInfo* __restrict__ cdom,
+                                                  const int* __restrict__ cl2n, const DomInfo* __restrict__ fdom,
+                                                  const int* __restrict__ fl2n, const float* __restrict__ rf,
+                                                  const int* __restrict__ done, float* __restrict__ bc) {
+    const BrickInfo B = cbr[blockIdx.x];
+    if (done && done[B.dom]) return;
+    const int tid = threadIdx.x;
+    if (tid >= T_NPP * T_LPN) return;
+    const DomInfo C = cdom[B.dom], F = fdom[B.dom];
+    const int slot = tid / T_LPN, j = tid % T_LPN;
+    const int cLx = C.nx + 1, cLy = C.ny + 1;
+    const int fLx = F.nx + 1, fLy = F.ny + 1;
+    const float4* rf4 = reinterpret_cast<const float4*>(rf);
+    float4* bc4 = reinterpret_cast<float4*>(bc);
+    for (int o = slot; o < NO; o += T_NPP) {
+        const int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
+        if (gx > C.nx || gy > C.ny || gz > C.nz) continue;
+        const int qc = cl2n[C.lat_off + gx + (int64_t)cLx * (gy + (int64_t)cLy * gz)];
+        if (qc < 0) continue;
+        float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!is_bc(C, gx, gy, gz)) {
+#pragma unroll
+            for (int k = -1; k <= 1; ++k)
+#pragma unroll
+                for (int jj = -1; jj <= 1; ++jj)
+#pragma unroll
+                    for (int i = -1; i <= 1; ++i) {
+                        const int X = 2 * gx + i, Y = 2 * gy + jj, Z = 2 * gz + k;
+                        if (X > 0 && Y > 0 && Z > 0 && X < F.nx && Y < F.ny && Z < F.nz) {   // free fine node
+                            const int qf = fl2n[F.lat_off + X + (int64_t)fLx * (Y + (int64_t)fLy * Z)];
+                            if (qf >= 0) {
+                                constexpr float W[3] = {0.5f, 1.f, 0.5f};
+                                const float w = W[i + 1] * W[jj + 1] * W[k + 1];
+                                f4_axpy(sum, w, rf4[(int64_t)qf * T_LPN + j]);
+                            }
+                        }
+                    }
+        }
+        bc4[(int64_t)qc * T_LPN + j] = sum;
+    }
+}
+
+// x_f += P x_c on the free nodes of each fine brick (trilinear interpolation)
+__global__ __launch_bounds__(256) void k_prolong(const BrickInfo* __restrict__ fbr, const DomInfo* __restrict__ fdom,
+                                                 const int* __restrict__ fl2n, const DomInfo* __restrict__ cdom,
+                                                 const int* __restrict__ cl2n, const float* __restrict__ xc,
+                                                 const int* __restrict__ done, float* __restrict__ xf) {
+    const BrickInfo B = fbr[blockIdx.x];
+    if (done && done[B.dom]) return;
+    const int tid = threadIdx.x;
+    if (tid >= T_NPP * T_LPN) return;
+    const DomInfo F = fdom[B.dom], C = cdom[B.dom];
+    const int slot = tid / T_LPN, j = tid % T_LPN;
+    const int fLx = F.nx + 1, fLy = F.ny + 1;
+    const int cLx = C.nx + 1, cLy = C.ny + 1;
+    const float4* xc4 = reinterpret_cast<const float4*>(xc);
+    float4* xf4 = reinterpret_cast<float4*>(xf);
+    for (int o = slot; o < NO; o += T_NPP) {
+        const int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
+        if (!(gx > 0 && gy > 0 && gz > 0 && gx < F.nx && gy < F.ny && gz < F.nz)) continue;   // free nodes only
+        const int qf = fl2n[F.lat_off + gx + (int64_t)fLx * (gy + (int64_t)fLy * gz)];
+        if (qf < 0) continue;
+        const int X0 = gx >> 1, Y0 = gy >> 1, Z0 = gz >> 1;
+        const int ox = gx & 1, oy = gy & 1, oz = gz & 1;
+        const float w = (ox ? 0.5f : 1.f) * (oy ? 0.5f : 1.f) * (oz ? 0.5f : 1.f);
+        float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    if ((i && !ox) || (jj && !oy) || (k && !oz)) continue;
+                    const int qc = cl2n[C.lat_off + (X0 + i) + (int64_t)cLx * ((Y0 + jj) + (int64_t)cLy * (Z0 + k))];
+                    if (qc >= 0) f4_axpy(sum, 1.f, xc4[(int64_t)qc * T_LPN + j]);
+                }
+        float4 v = xf4[(int64_t)qf * T_LPN + j];
+        f4_axpy(v, w, sum);
+        xf4[(int64_t)qf * T_LPN + j] = v;
+    }
+}
+
+// one Chebyshev-Jacobi step on the owned nodes of each brick:
+//   zero:      d = c2 D^-1 b,              x = d        (first step from x = 0; t unused)
+//   otherwise: d = c1 d + c2 D^-1 (b - t), x = x + d    (t = A x)
+// prz (optional): per-brick sums of b . x per load case
+__global__ __launch_bounds__(NT_UPD) void k_cheb(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                                 const float* __restrict__ b, const float* __restrict__ t,
+                                                 const float* __restrict__ dinv, float* __restrict__ d,
+                                                 float* __restrict__ x, float c1, float c2, int zero,
+                                                 double* __restrict__ prz) {
+    __shared__ double s1[NT_UPD];
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done && done[B.dom]) return;
+    const int tid = threadIdx.x;
+    double a1 = 0.0;
+    const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
+    for (int64_t i = tid; i < cnt; i += NT_UPD) {
+        const int64_t idx = base + i;
+        const float bv = b[idx];
+        const float di = dinv[idx / (3 * R)];
+        float dv, xv;
+        if (zero) {
+            dv = c2 * di * bv;
+            xv = dv;
+        } else {
+            dv = c1 * d[idx] + c2 * di * (bv - t[idx]);
+            xv = x[idx] + dv;
+        }
+        d[idx] = dv;
+        x[idx] = xv;
+        a1 += (double)bv * (double)xv;
+    }
+    if (prz) {
+        s1[tid] = a1;
+        __syncthreads();
+        if (tid < R) {
+            double u = 0.0;
+            for (int k = tid; k < NT_UPD; k += R) u += s1[k];
+            prz[(int64_t)blockIdx.x * R + tid] = u;
+        }
+    }
 }
 
 inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -1081,6 +1414,70 @@ int batch_add_domain(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d,
     return DTC_OK;
 }
 
+namespace {
+
+// largest eigenvalue of a symmetric positive semidefinite n x n matrix (power iteration)
+double sym_lmax(const double* A, int n) {
+    std::vector<double> v(n, 1.0), w(n);
+    for (int i = 0; i < n; ++i) v[i] += 0.01 * i;
+    double lam = 0.0;
+    for (int it = 0; it < 2000; ++it) {
+        double nrm = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double t = 0.0;
+            for (int j = 0; j < n; ++j) t += A[i + n * j] * v[j];
+            w[i] = t;
+            nrm += t * t;
+        }
+        nrm = std::sqrt(nrm);
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < n; ++i) { num += v[i] * w[i]; den += v[i] * v[i]; }
+        lam = num / den;
+        for (int i = 0; i < n; ++i) v[i] = w[i] / nrm;
+    }
+    return lam;
+}
+
+// matvec tables of a brick list: tile element counts -> offsets -> halo table,
+// owned-node element masks and packed element lists (bricks re-uploaded with offsets)
+int build_tables(hipStream_t s, std::vector<BrickInfo>& hb, DevMem& brick, const DevMem& dom, const DevMem& lat2node,
+                 const DevMem& emod, DevMem& halo, DevMem& nmask, DevMem& einfo, DevMem& eE64, DevMem& eE32,
+                 int64_t* n_tile) {
+    const int nbr = (int)hb.size();
+    *n_tile = 0;
+    if (nbr == 0) return DTC_OK;
+    DevMem cnt;
+    DTC_TRY(cnt.alloc((size_t)nbr * 4));
+    k_brick_ecount<<<nbr, 256, 0, s>>>(brick.as<BrickInfo>(), dom.as<DomInfo>(), emod.as<double>(), cnt.as<int>());
+    DTC_HIP(hipGetLastError());
+    std::vector<int> hc(nbr);
+    DTC_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)nbr * 4, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipStreamSynchronize(s));
+    int64_t off = 0;
+    for (int i = 0; i < nbr; ++i) {
+        hb[i].e_off = (int)off;
+        hb[i].e_cnt = hc[i];
+        off += hc[i];
+    }
+    if (off >= INT32_MAX) return set_error(DTC_ERR_BAD_ARGUMENT, "batch exceeds %d tile elements", INT32_MAX);
+    *n_tile = off;
+    DTC_HIP(hipMemcpyAsync(brick.p, hb.data(), hb.size() * sizeof(BrickInfo), hipMemcpyHostToDevice, s));
+    const size_t ne = (size_t)(off > 0 ? off : 1);
+    DTC_TRY(halo.ensure((size_t)nbr * NH * 4));
+    DTC_TRY(nmask.ensure((size_t)nbr * NO));
+    DTC_TRY(einfo.ensure(ne * 4));
+    DTC_TRY(eE64.ensure(ne * 8));
+    DTC_TRY(eE32.ensure(ne * 4));
+    k_brick_tables<<<nbr, 256, 0, s>>>(brick.as<BrickInfo>(), dom.as<DomInfo>(), lat2node.as<int>(), emod.as<double>(),
+                                       halo.as<int>(), nmask.as<unsigned char>(), einfo.as<int>(), eE64.as<double>(),
+                                       eE32.as<float>());
+    DTC_HIP(hipGetLastError());
+    DTC_HIP(hipStreamSynchronize(s));   // cnt goes out of scope
+    return DTC_OK;
+}
+
+}  // namespace
+
 int batch_finalize(SolverBatch& b, int precision) {
     b.n_bricks = (int)b.h_brick.size();
     hipStream_t s = b.stream;
@@ -1123,13 +1520,123 @@ int batch_finalize(SolverBatch& b, int precision) {
     DTC_TRY(element_stiffness(8, 1.0, b.nu, b.a, Khat));
     DTC_TRY(walsh_blocks(Khat, &b.wk64));
     b.kdiag = Khat[0];
+    // element-by-element bound lambda_max(D^-1 K) <= lambda_max(diag(Khat)^-1/2 Khat diag(Khat)^-1/2);
+    // the diagonal of the cube element is constant
+    b.cheb_lmax = sym_lmax(Khat, 24) / b.kdiag;
     if (b.n_bricks > 0) {
         k_dinv<<<b.n_bricks, BX * BY * BZ, 0, s>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
                                                    b.emod.as<double>(), b.kdiag, b.dinv64.as<double>(), b.dinv32.as<float>());
         DTC_HIP(hipGetLastError());
+        DTC_TRY(build_tables(s, b.h_brick, b.brick, b.dom, b.lat2node, b.emod, b.halo, b.nmask, b.einfo, b.eE64,
+                             b.eE32, &b.n_tile_elems));
     }
     DTC_HIP(hipStreamSynchronize(s));
     b.precision = precision;
+    return DTC_OK;
+}
+
+int batch_build_mg(SolverBatch& b, int max_levels) {
+    hipStream_t s = b.stream;
+    int maxn = 0;
+    for (const DomInfo& D : b.h_dom) maxn = std::max(maxn, std::max(D.nx, std::max(D.ny, D.nz)));
+    int L = 0;
+    for (int n = maxn; n > 8 && L < max_levels; n = (n + 1) / 2) ++L;
+    b.mg.resize(L);
+    const size_t v0 = (size_t)(b.n_nodes > 0 ? b.n_nodes : 1) * 3 * R * 4;
+    DTC_TRY(b.z32.ensure(v0));
+    DTC_TRY(b.t32.ensure(v0));
+    DTC_TRY(b.d32.ensure(v0));
+    for (int l = 0; l < L; ++l) {
+        Level& C = b.mg[l];
+        const std::vector<DomInfo>& FD = l == 0 ? b.h_dom : b.mg[l - 1].h_dom;
+        const double* fe = l == 0 ? b.emod.as<double>() : b.mg[l - 1].emod.as<double>();
+        C.h_dom.assign(FD.begin(), FD.end());
+        int64_t nlat = 0, nvox = 0, nbt = 0;
+        std::vector<int64_t> boff(C.h_dom.size());
+        for (size_t d = 0; d < C.h_dom.size(); ++d) {
+            DomInfo& D = C.h_dom[d];
+            D.nx = (D.nx + 1) / 2; D.ny = (D.ny + 1) / 2; D.nz = (D.nz + 1) / 2;
+            for (int k = 0; k < 3; ++k) D.delta[k] *= 2.0;
+            D.bnx = (D.nx + BX) / BX; D.bny = (D.ny + BY) / BY; D.bnz = (D.nz + BZ) / BZ;
+            D.lat_off = nlat;
+            D.vox_off = nvox;
+            boff[d] = nbt;
+            nlat += (int64_t)(D.nx + 1) * (D.ny + 1) * (D.nz + 1);
+            nvox += (int64_t)D.nx * D.ny * D.nz;
+            nbt += (int64_t)D.bnx * D.bny * D.bnz;
+        }
+        C.n_lat = nlat;
+        C.n_vox = nvox;
+        DTC_TRY(C.emod.ensure((size_t)std::max<int64_t>(nvox, 1) * 8));
+        DTC_TRY(C.lat2node.ensure((size_t)std::max<int64_t>(nlat, 1) * 4));
+        DevMem ex, cnt;
+        DTC_TRY(ex.alloc((size_t)std::max<int64_t>(nlat, 1) * 4));
+        DTC_TRY(cnt.alloc((size_t)std::max<int64_t>(nbt, 1) * 4));
+        k_fill_i32b<<<nb(nlat, 256), 256, 0, s>>>(C.lat2node.as<int>(), nlat, -1);
+        for (size_t d = 0; d < C.h_dom.size(); ++d) {
+            const DomInfo &F = FD[d], &D = C.h_dom[d];
+            const int64_t cv = (int64_t)D.nx * D.ny * D.nz, cl = (int64_t)(D.nx + 1) * (D.ny + 1) * (D.nz + 1);
+            k_coarsen<<<nb(cv, 256), 256, 0, s>>>(fe + F.vox_off, F.nx, F.ny, F.nz, C.emod.as<double>() + D.vox_off,
+                                                 D.nx, D.ny, D.nz);
+            k_exist<<<nb(cl, 256), 256, 0, s>>>(C.emod.as<double>() + D.vox_off, D.nx, D.ny, D.nz,
+                                               ex.as<int>() + D.lat_off);
+            k_brick_count_ex<<<D.bnx * D.bny * D.bnz, NO, 0, s>>>(ex.as<int>() + D.lat_off, D.nx + 1, D.ny + 1,
+                                                                   D.nz + 1, D.bnx, D.bny, cnt.as<int>() + boff[d]);
+        }
+        DTC_HIP(hipGetLastError());
+        std::vector<int> hc((size_t)std::max<int64_t>(nbt, 1));
+        DTC_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)nbt * 4, hipMemcpyDeviceToHost, s));
+        DTC_HIP(hipStreamSynchronize(s));
+        C.h_brick.clear();
+        int64_t node = 0;
+        for (size_t d = 0; d < C.h_dom.size(); ++d) {
+            DomInfo& D = C.h_dom[d];
+            D.brick_begin = (int)C.h_brick.size();
+            D.node_begin = node;
+            const int nbr = D.bnx * D.bny * D.bnz;
+            for (int i = 0; i < nbr; ++i) {
+                const int c = hc[boff[d] + i];
+                if (!c) continue;
+                BrickInfo B{};
+                B.dom = (int)d;
+                B.x0 = (i % D.bnx) * BX; B.y0 = ((i / D.bnx) % D.bny) * BY; B.z0 = (i / (D.bnx * D.bny)) * BZ;
+                B.node_begin = (int)node;
+                node += c;
+                B.node_end = (int)node;
+                C.h_brick.push_back(B);
+            }
+            D.brick_end = (int)C.h_brick.size();
+            D.node_end = node;
+        }
+        C.n_bricks = (int)C.h_brick.size();
+        C.n_nodes = node;
+        DTC_TRY(C.dom.ensure(C.h_dom.size() * sizeof(DomInfo)));
+        DTC_HIP(hipMemcpyAsync(C.dom.p, C.h_dom.data(), C.h_dom.size() * sizeof(DomInfo), hipMemcpyHostToDevice, s));
+        DTC_TRY(C.brick.ensure((C.h_brick.size() + 1) * sizeof(BrickInfo)));
+        if (C.n_bricks > 0)
+            DTC_HIP(hipMemcpyAsync(C.brick.p, C.h_brick.data(), C.h_brick.size() * sizeof(BrickInfo),
+                                   hipMemcpyHostToDevice, s));
+        for (size_t d = 0; d < C.h_dom.size(); ++d) {
+            const DomInfo& D = C.h_dom[d];
+            if (D.brick_end > D.brick_begin)
+                k_brick_assign_ex<<<D.brick_end - D.brick_begin, NO, 0, s>>>(
+                    C.brick.as<BrickInfo>() + D.brick_begin, ex.as<int>() + D.lat_off, D.nx + 1, D.ny + 1, D.nz + 1,
+                    D.lat_off, C.lat2node.as<int>());
+        }
+        DTC_HIP(hipGetLastError());
+        const size_t nn = (size_t)(C.n_nodes > 0 ? C.n_nodes : 1);
+        DTC_TRY(C.dinv64.ensure(nn * 8));
+        DTC_TRY(C.dinv32.ensure(nn * 4));
+        for (DevMem* m : {&C.b, &C.x, &C.t, &C.d}) DTC_TRY(m->ensure(nn * 3 * R * 4));
+        if (C.n_bricks > 0) {
+            k_dinv<<<C.n_bricks, NO, 0, s>>>(C.brick.as<BrickInfo>(), C.dom.as<DomInfo>(), C.lat2node.as<int>(),
+                                            C.emod.as<double>(), b.kdiag, C.dinv64.as<double>(), C.dinv32.as<float>());
+            DTC_HIP(hipGetLastError());
+            DTC_TRY(build_tables(s, C.h_brick, C.brick, C.dom, C.lat2node, C.emod, C.halo, C.nmask, C.einfo, C.eE64,
+                                 C.eE32, &C.n_tile_elems));
+        }
+        DTC_HIP(hipStreamSynchronize(s));   // ex, cnt go out of scope
+    }
     return DTC_OK;
 }
 
@@ -1168,7 +1675,8 @@ int harvest_events(SolverBatch& b) {
 template <typename T, int IN, int OUT>
 int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* part, const int* done) {
     if (b.n_bricks == 0) return DTC_OK;
-    MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(), b.emod.as<double>(), done, part};
+    MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(), b.einfo.as<int>(),
+             sizeof(T) == 4 ? b.eE32.p : b.eE64.p, done, part};
     // the CG matvec is the dominant kernel: bracket it with events on the batch stream
     const bool t = b.timing && IN == IN_V && OUT == OUT_MASKED_DOT;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1182,20 +1690,161 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
     return DTC_OK;
 }
 
-// one PCG iteration on (x, r, p, q) in precision T
+// topology of one level for the FP32 matvec
+struct MvTopo {
+    int n_bricks;
+    const BrickInfo* bricks;
+    const DomInfo* doms;
+    const int* halo;
+    const unsigned char* nmask;
+    const int* einfo;
+    const void* eE32;
+};
+
+template <int OUT>
+int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float* x, float* y, const int* done,
+                const float* bv = nullptr, const float* dinv = nullptr, float* dv = nullptr, float c1 = 0.f,
+                float c2 = 0.f, double* partial = nullptr) {
+    if (T.n_bricks == 0) return DTC_OK;
+    MvArgs a{T.bricks, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2};
+    k_matvec<float, IN_V, OUT><<<T.n_bricks, MV_NT, 0, s>>>(a, wk, x, y);
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+// vectors and topology of multigrid level l (0 = the fine level, whose right-hand
+// side is the PCG residual); the iterate alternates between buf[0] and buf[1]
+struct LvView {
+    MvTopo T;
+    const int* l2n;
+    const float* dinv;
+    float* b;
+    float* buf[2];
+    float* d;
+};
+
+// Chebyshev smoothing parameters: interval [lmax / ratio, lmax] of D^-1 K
+struct MgParams {
+    double ratio = 10.0;       // smoothing passes of the fine and intermediate levels
+    int coarse_degree = 8;     // coarsest level
+    double coarse_ratio = 50.0;
+};
+
+MgParams mg_params() {
+    MgParams p;
+    if (const char* e = std::getenv("DTC_MG_RATIO")) p.ratio = std::atof(e);
+    if (const char* e = std::getenv("DTC_MG_CDEG")) p.coarse_degree = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("DTC_MG_CRATIO")) p.coarse_ratio = std::atof(e);
+    return p;
+}
+
+// z = V(r): one V-cycle over the levels with Chebyshev-Jacobi pre- and
+// post-smoothing by the same polynomial (a symmetric preconditioner), trilinear
+// prolongation and its transpose as restriction.  Every smoothing step after the
+// first is one fused matvec (OUT_CHEB); the residual before restriction is one
+// OUT_BRES matvec.  prz (optional) receives per-brick sums of r . z.
+int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, double* prz) {
+    hipStream_t s = b.stream;
+    const int L = (int)b.mg.size();
+    const int* done = b.done.as<int>();
+    static const MgParams P = mg_params();
+    const int deg = b.mg_degree > 0 ? b.mg_degree : 2;
+    auto view = [&](int l) {
+        LvView v;
+        if (l == 0) {
+            v.T = {b.n_bricks, b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(),
+                   b.nmask.as<unsigned char>(), b.einfo.as<int>(), b.eE32.p};
+            v.l2n = b.lat2node.as<int>();
+            v.dinv = b.dinv32.as<float>();
+            v.b = const_cast<float*>(r);
+            v.buf[0] = z;
+            v.buf[1] = b.t32.as<float>();
+            v.d = b.d32.as<float>();
+        } else {
+            Level& C = b.mg[l - 1];
+            v.T = {C.n_bricks, C.brick.as<BrickInfo>(), C.dom.as<DomInfo>(), C.halo.as<int>(),
+                   C.nmask.as<unsigned char>(), C.einfo.as<int>(), C.eE32.p};
+            v.l2n = C.lat2node.as<int>();
+            v.dinv = C.dinv32.as<float>();
+            v.b = C.b.as<float>();
+            v.buf[0] = C.x.as<float>();
+            v.buf[1] = C.t.as<float>();
+            v.d = C.d.as<float>();
+        }
+        return v;
+    };
+    std::vector<int> cur(L + 1, 0);
+    auto smooth = [&](int l, int m, bool zero, double ratio, double* dot) -> int {
+        const LvView v = view(l);
+        if (v.T.n_bricks == 0) return DTC_OK;
+        const double hi = b.cheb_lmax, lo = hi / ratio;
+        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+        double rho = 1.0 / sigma;
+        for (int k = 0; k < m; ++k) {
+            float c1 = 0.f, c2 = (float)(1.0 / theta);
+            if (k > 0) {
+                const double rn = 1.0 / (2.0 * sigma - rho);
+                c1 = (float)(rn * rho);
+                c2 = (float)(2.0 * rn / delta);
+                rho = rn;
+            }
+            double* dk = k == m - 1 ? dot : nullptr;
+            if (zero && k == 0) {
+                k_cheb<<<v.T.n_bricks, NT_UPD, 0, s>>>(v.T.bricks, done, v.b, nullptr, v.dinv, v.d, v.buf[cur[l]], c1,
+                                                       c2, 1, dk);
+                DTC_HIP(hipGetLastError());
+            } else {
+                DTC_TRY(launch_mv32<OUT_CHEB>(v.T, s, wk, v.buf[cur[l]], v.buf[1 - cur[l]], done, v.b, v.dinv, v.d,
+                                              c1, c2, dk));
+                cur[l] = 1 - cur[l];
+            }
+        }
+        return DTC_OK;
+    };
+    // the fine iterate makes (deg - 1) + deg swaps (or coarse_degree - 1 without
+    // coarse levels): start in the buffer that makes it end in z
+    cur[0] = (L > 0 ? 2 * deg - 1 : P.coarse_degree - 1) % 2;
+    for (int l = 0; l < L; ++l) {
+        DTC_TRY(smooth(l, deg, true, P.ratio, nullptr));
+        const LvView f = view(l), c = view(l + 1);
+        cur[l + 1] = 0;
+        if (c.T.n_bricks == 0 || f.T.n_bricks == 0) continue;
+        float* res = f.buf[1 - cur[l]];
+        DTC_TRY(launch_mv32<OUT_BRES>(f.T, s, wk, f.buf[cur[l]], res, done, f.b));
+        k_restrict<<<c.T.n_bricks, 256, 0, s>>>(c.T.bricks, c.T.doms, c.l2n, f.T.doms, f.l2n, res, done, c.b);
+        DTC_HIP(hipGetLastError());
+    }
+    DTC_TRY(smooth(L, P.coarse_degree, true, P.coarse_ratio, L == 0 ? prz : nullptr));
+    for (int l = L - 1; l >= 0; --l) {
+        const LvView f = view(l), c = view(l + 1);
+        if (f.T.n_bricks > 0 && c.T.n_bricks > 0) {
+            k_prolong<<<f.T.n_bricks, 256, 0, s>>>(f.T.bricks, f.T.doms, f.l2n, c.T.doms, c.l2n, c.buf[cur[l + 1]], done,
+                                                   f.buf[cur[l]]);
+            DTC_HIP(hipGetLastError());
+        }
+        DTC_TRY(smooth(l, deg, false, P.ratio, l == 0 ? prz : nullptr));
+    }
+    return DTC_OK;
+}
+
+// one PCG iteration on (x, r, p, q) in precision T; zmg != null: multigrid
+// preconditioner (FP32 only), z = V(r) stored in zmg
 template <typename T>
-int pcg_iteration(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* dinv, int64_t max_iters) {
+int pcg_iteration(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* dinv, int64_t max_iters, T* zmg) {
     const int D = b.n_dom, NBk = b.n_bricks;
     int* done = b.done.as<int>();
     DTC_TRY((launch_mv<T, IN_V, OUT_MASKED_DOT>(b, wk, p, q, b.part_a.as<double>(), done)));
     k_red_alpha<<<D, RED_NT, 0, b.stream>>>(b.dom.as<DomInfo>(), done, b.part_a.as<double>(), b.active.as<int>(),
                                         b.rz.as<double>(), b.alpha.as<double>());
     k_cg_update<T><<<NBk, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), done, b.alpha.as<double>(), x, r, p, q, dinv,
-                                                 b.part_a.as<double>(), b.part_b.as<double>());
+                                                 zmg ? nullptr : b.part_a.as<double>(), b.part_b.as<double>());
+    if constexpr (std::is_same<T, float>::value) {
+        if (zmg) DTC_TRY(mg_vcycle(b, wk, r, zmg, b.part_a.as<double>()));
+    }
     k_red_beta<<<D, RED_NT, 0, b.stream>>>(b.dom.as<DomInfo>(), done, b.part_a.as<double>(), b.part_b.as<double>(),
                                        b.thr2.as<double>(), b.active.as<int>(), b.rz.as<double>(), b.beta.as<double>(),
                                        b.iters.as<int64_t>(), max_iters);
-    k_cg_dir<T><<<NBk, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), done, b.beta.as<double>(), r, p, dinv);
+    k_cg_dir<T><<<NBk, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), done, b.beta.as<double>(), r, p, dinv, zmg);
     DTC_HIP(hipGetLastError());
     return DTC_OK;
 }
@@ -1213,13 +1862,13 @@ int all_done(SolverBatch& b, const DevMem& flags, bool* out) {
 
 template <typename T>
 int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* dinv, int64_t max_iters,
-            int64_t max_steps) {
-    constexpr int CHECK = 20;
+            int64_t max_steps, T* zmg) {
+    const int CHECK = zmg ? 4 : 20;
     for (int64_t steps = 0; steps < max_steps; steps += CHECK) {
         bool fin = false;
         DTC_TRY(all_done(b, b.done, &fin));
         if (fin) return DTC_OK;
-        for (int k = 0; k < CHECK; ++k) DTC_TRY(pcg_iteration<T>(b, wk, x, r, p, q, dinv, max_iters));
+        for (int k = 0; k < CHECK; ++k) DTC_TRY(pcg_iteration<T>(b, wk, x, r, p, q, dinv, max_iters, zmg));
     }
     return DTC_OK;
 }
@@ -1228,8 +1877,9 @@ int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* di
 // then per load case activity and targets (k_red_outer)
 int residual64(SolverBatch& b, const WK<double>& wk, const double* x64, int first, const dtc_solve_opts& o) {
     DTC_TRY((launch_mv<double, IN_U, OUT_RESID>(b, wk, x64, b.r64.as<double>(), b.part_c.as<double>(), nullptr)));
+    const double itol2 = b.precision == DTC_FP32_REFINE ? o.inner_tol * o.inner_tol : 0.0;
     k_red_outer<<<b.n_dom, RED_NT, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_c.as<double>(), first,
-                                                  o.rel_tol * o.rel_tol, b.rr0.as<double>(), b.rr.as<double>(),
+                                                  o.rel_tol * o.rel_tol, itol2, b.rr0.as<double>(), b.rr.as<double>(),
                                                   b.active64.as<int>(), b.active.as<int>(), b.thr2.as<double>(),
                                                   b.done64.as<int>(), b.done.as<int>());
     DTC_HIP(hipGetLastError());
@@ -1300,38 +1950,44 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
             double* r = b.r64.as<double>();
             k_cg_init<double><<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.active.as<int>(), x, r,
                                                             b.p64.as<double>(), b.dinv64.as<double>(),
-                                                            b.part_a.as<double>(), b.part_b.as<double>(), 0, 1, nullptr);
+                                                            b.part_a.as<double>(), b.part_b.as<double>(), 0, 1, nullptr,
+                                                            nullptr);
             k_red_init<<<D, RED_NT, 0, s>>>(dl, b.done64.as<int>(), b.part_a.as<double>(), b.part_b.as<double>(),
                                             b.thr2.as<double>(), b.active.as<int>(), b.rz.as<double>(), b.done.as<int>());
             DTC_HIP(hipGetLastError());
             DTC_TRY(pcg_run<double>(b, wk64, x, r, b.p64.as<double>(), b.q64.as<double>(), b.dinv64.as<double>(), cap,
-                                    INT64_MAX));
+                                    INT64_MAX, nullptr));
             DTC_TRY(residual64(b, wk64, b.x64.as<double>(), 0, o));
         }
     } else {
         // FP32 PCG with FP64 residual replacement: the FP32 correction x32 is
-        // folded into x64 every REPLACE iterations (and when the recursive
-        // residual meets its target), r32 is replaced by the true FP64
-        // residual and the search direction p32 is kept.
-        constexpr int64_t REPLACE = 500;
+        // folded into x64 when the recursive residual has dropped by inner_tol
+        // (or meets the final target, or after REPLACE iterations), r32 is
+        // replaced by the true FP64 residual and the search direction p32 is
+        // kept.
+        const bool mg = b.precond == DTC_PC_MG;
+        const int64_t REPLACE = mg ? 100 : 1000;
         float* x32 = b.x32.as<float>();
         float* r32 = b.r32.as<float>();
         float* p32 = b.p32.as<float>();
+        float* z32 = mg ? b.z32.as<float>() : nullptr;
         bool first = true;
         for (int it = 0; it < 100000; ++it) {
             bool left = false;
             DTC_TRY(apply_cap(b, cap, nouter, &left));
             if (!left) break;
             k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.r64.as<double>(), r32, x32);
+            if (mg) DTC_TRY(mg_vcycle(b, wk32, r32, z32, nullptr));
             k_cg_init<float><<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.active.as<int>(), x32, r32, p32,
                                                            b.dinv32.as<float>(), b.part_a.as<double>(),
                                                            b.part_b.as<double>(), 0, first ? 1 : 0,
-                                                           b.active_prev.as<int>());
+                                                           b.active_prev.as<int>(), z32);
             k_red_init<<<D, RED_NT, 0, s>>>(dl, b.done64.as<int>(), b.part_a.as<double>(), b.part_b.as<double>(),
                                             b.thr2.as<double>(), b.active.as<int>(), b.rz.as<double>(), b.done.as<int>());
             DTC_HIP(hipGetLastError());
             first = false;
-            DTC_TRY(pcg_run<float>(b, wk32, x32, r32, p32, b.q32.as<float>(), b.dinv32.as<float>(), cap, REPLACE));
+            DTC_TRY(pcg_run<float>(b, wk32, x32, r32, p32, b.q32.as<float>(), b.dinv32.as<float>(), cap, REPLACE,
+                                   z32));
             DTC_HIP(hipMemcpyAsync(b.active_prev.p, b.active.p, (size_t)D * R * 4, hipMemcpyDeviceToDevice, s));
             k_acc64<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), x32, b.x64.as<double>());
             DTC_HIP(hipGetLastError());
@@ -1350,10 +2006,14 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
         iters[d] = hi[d];
         outer[d] = nouter[d];
         double mx = 0.0;
-        for (int lc = 0; lc < R; ++lc)
-            if (hr0[d * R + lc] > 0.0) mx = std::fmax(mx, std::sqrt(hrr[d * R + lc] / hr0[d * R + lc]));
-        relres[d] = mx;
-        if (b.h_status[d] == DTC_OK && mx > o.rel_tol) b.h_status[d] = DTC_ERR_NO_CONVERGENCE;
+        bool finite = true;
+        for (int lc = 0; lc < R; ++lc) {
+            const double r0 = hr0[d * R + lc], rr = hrr[d * R + lc];
+            if (!std::isfinite(r0) || !std::isfinite(rr)) finite = false;
+            else if (r0 > 0.0) mx = std::fmax(mx, std::sqrt(rr / r0));
+        }
+        relres[d] = finite ? mx : NAN;
+        if (b.h_status[d] == DTC_OK && !(finite && mx <= o.rel_tol)) b.h_status[d] = DTC_ERR_NO_CONVERGENCE;
     }
     return DTC_OK;
 }

@@ -4,6 +4,17 @@
 
 namespace dtc {
 
+// One coarse level of the multigrid preconditioner: voxels of 2^l fine voxels,
+// same brick layout and matvec tables as the fine level, FP32 vectors.
+struct Level {
+    int n_bricks = 0;
+    int64_t n_nodes = 0, n_lat = 0, n_vox = 0, n_tile_elems = 0;
+    std::vector<DomInfo> h_dom;
+    std::vector<BrickInfo> h_brick;
+    DevMem dom, brick, lat2node, emod, halo, nmask, einfo, eE64, eE32, dinv64, dinv32;
+    DevMem b, x, t, d;             // right-hand side, iterate, matvec output, Chebyshev direction
+};
+
 // One batch of domains resident on the device: topology, vectors, scalars.
 struct SolverBatch {
     int n_dom = 0;
@@ -18,6 +29,12 @@ struct SolverBatch {
     DevMem lat2node;               // int32 per lattice node (-1 = no solver node)
     DevMem sol2ref;                // int32 per solver node: mesh node id (1-based)
     DevMem emod;                   // double per voxel (0 = no element)
+    // per-brick matvec tables, built once per solve
+    DevMem halo;                   // int [brick][HX*HY*HZ]: solver node | BC_BIT, -1 = none
+    DevMem nmask;                  // uint8 [brick][BX*BY*BZ]: solid tile elements around each owned node
+    DevMem einfo;                  // int per tile element: halo base | (owned base + 41) << 9 | owned corners << 24
+    DevMem eE64, eE32;             // element modulus per tile element
+    int64_t n_tile_elems = 0;
     DevMem dinv64, dinv32;         // Jacobi: 1 / (kdiag * sum of adjacent moduli)
     // vectors, layout [node][3][24]
     DevMem x64, r64, p64, q64;
@@ -34,6 +51,12 @@ struct SolverBatch {
     DevMem iters;                  // int64 [dom]
     DevMem ff;                     // double [dom][24 lc][24]
     DevMem gtmp;                   // gradient work
+    // multigrid preconditioner of the FP32 PCG (empty when Jacobi is used)
+    std::vector<Level> mg;
+    DevMem z32, t32, d32;          // fine-level V-cycle output, matvec output, Chebyshev direction
+    int precond = 0;               // DTC_PC_JACOBI / DTC_PC_MG
+    int mg_degree = 2;             // Chebyshev degree of the fine and intermediate levels
+    double cheb_lmax = 0.0;        // element-by-element bound of lambda_max(D^-1 K)
     WalshK wk64{};
     double kdiag = 0.0;            // diagonal entry of the E = 1 element stiffness
     double nu = 0.3, a = 1.0;
@@ -64,6 +87,8 @@ int batch_add_domain(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d,
 int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status);
 // Allocate vectors and scalars once all domains are added.
 int batch_finalize(SolverBatch& b, int precision);
+// Build the coarse levels of the multigrid preconditioner (after batch_finalize).
+int batch_build_mg(SolverBatch& b, int max_levels);
 // Full solve: returns per-domain status in b.h_status; fills iteration counts.
 int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& iters,
                 std::vector<int64_t>& outer, std::vector<double>& relres);

@@ -25,7 +25,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (DTC_ERR_EMPTY_MESH, DTC_ERR_FEW_CONSTRAINTS, DTC_ERR_NO_CONVERGENCE, DTC_FP32_REFINE, DTC_FP64,
-                   DTC_OK, DTCError, DomainDesc, DomainResult, SolveOpts, check, load)
+                   DTC_PC_JACOBI, DTC_PC_MG, DTC_OK, DTCError, DomainDesc, DomainResult, SolveOpts, check, load)
 
 KIND = {np.dtype(np.int8): 1, np.dtype(np.int16): 2, np.dtype(np.int32): 4}
 
@@ -182,10 +182,11 @@ class Domain:
 
 
 def opts(precision=DTC_FP32_REFINE, rel_tol=1e-8, inner_tol=1e-4, max_iter=200000, rotation=True, rot_step_deg=1.0,
-         num_zero=1e-9) -> SolveOpts:
+         num_zero=1e-9, precond=DTC_PC_JACOBI, mg_degree=0) -> SolveOpts:
     o = SolveOpts()
     o.precision, o.rotation, o.rel_tol, o.inner_tol = int(precision), int(bool(rotation)), float(rel_tol), float(inner_tol)
     o.max_iter, o.rot_step_deg, o.num_zero = int(max_iter), float(rot_step_deg), float(num_zero)
+    o.precond, o.mg_degree = int(precond), int(mg_degree)
     return o
 
 

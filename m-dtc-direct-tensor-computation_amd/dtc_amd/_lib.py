@@ -22,6 +22,8 @@ DTC_ERR_DEVICE = 6
 DTC_ERR_UNSUPPORTED = 7
 DTC_FP64 = 0
 DTC_FP32_REFINE = 1
+DTC_PC_JACOBI = 0
+DTC_PC_MG = 1
 
 I64 = C.c_int64
 P = C.c_void_p
@@ -41,6 +43,7 @@ class SolveOpts(C.Structure):
     _fields_ = [
         ("precision", C.c_int), ("rotation", C.c_int), ("rel_tol", C.c_double), ("inner_tol", C.c_double),
         ("max_iter", I64), ("rot_step_deg", C.c_double), ("num_zero", C.c_double),
+        ("precond", C.c_int), ("mg_degree", C.c_int),
     ]
 
 

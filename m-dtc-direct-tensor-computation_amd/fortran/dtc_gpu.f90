@@ -23,6 +23,7 @@ module dtc_gpu
        DTC_ERR_UNSUPPORTED = 7
   integer(c_int), parameter :: DTC_IK1 = 1, DTC_IK2 = 2, DTC_IK4 = 4
   integer(c_int), parameter :: DTC_FP64 = 0, DTC_FP32_REFINE = 1
+  integer(c_int), parameter :: DTC_PC_JACOBI = 0, DTC_PC_MG = 1
 
   type, bind(c) :: dtc_domain_desc
      type(c_ptr)          :: phi = c_null_ptr
@@ -48,6 +49,8 @@ module dtc_gpu
      integer(c_int64_t)   :: max_iter = 200000
      real(c_double)       :: rot_step_deg = 1._c_double
      real(c_double)       :: num_zero = 1.e-9_c_double
+     integer(c_int)       :: precond = DTC_PC_MG
+     integer(c_int)       :: mg_degree = 0
   end type dtc_solve_opts
 
   type, bind(c) :: dtc_domain_result

@@ -31,7 +31,7 @@ def test_struct_layouts_match_the_header():
     import dtc_amd._lib as L
     # sizes implied by the header (all members naturally aligned, no implicit padding)
     assert ctypes.sizeof(L.DomainDesc) == 8 + 4 * 4 + 3 * 8 + 8 + 3 * 8 + 3 * 8 + 8 + 3 * 8 + 8
-    assert ctypes.sizeof(L.SolveOpts) == 4 + 4 + 8 + 8 + 8 + 8 + 8
+    assert ctypes.sizeof(L.SolveOpts) == 4 + 4 + 8 + 8 + 8 + 8 + 8 + 4 + 4
     assert ctypes.sizeof(L.DomainResult) == 8 + 6 * 8 + 576 * 16 + 72 * 8 + 8 + 2 * 8 + 6 * 8 + 18 * 8 + 72 * 8 + 2 * 8 + 2 * 8
 
 
