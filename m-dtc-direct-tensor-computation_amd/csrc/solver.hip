@@ -206,23 +206,35 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
     // x loads run two chunks ahead: chunk c is staged from regs[c % 2]
     static_assert(NCH % 2 == 0, "chunk loop is unrolled by two");
     V regA[NLD], regB[NLD];
-    auto prefetch = [&](V (&reg)[NLD], int chunk, const int* nodes) {
+    // chunk-invariant 16-byte offsets of the halo segments this thread stages
+    int pf_off[NLD];
+#pragma unroll
+    for (int k = 0; k < NLD; ++k) {
+        const int i = tid + k * MV_NT;
+        int o = -1;
+        if (IN == IN_V && i < NH * 3) {
+            const int q = halo[i / 3];
+            if (q >= 0) o = ((q & NODE_MASK) * 3 + i % 3) * NCH;
+        }
+        pf_off[k] = o;
+    }
+    auto prefetch = [&](V (&reg)[NLD], int chunk) {
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
-            const int i = tid + k * MV_NT;
             V v;
             vzero(v);
-            if (i < NH * 3) {
-                const int q = nodes[i / 3];
-                if (q >= 0) v = xv[((int64_t)(q & NODE_MASK) * 3 + i % 3) * NCH + chunk];
-            }
+            if (pf_off[k] >= 0) v = xv[pf_off[k] + chunk];
             reg[k] = v;
         }
     };
     if (IN == IN_V) {
-        prefetch(regA, 0, halo);
-        prefetch(regB, 1, halo);
+        prefetch(regA, 0);
+        prefetch(regB, 1);
     }
+    // chunk-invariant gather items of this thread: owned (node, direction) i = tid + j * MV_NT
+    constexpr int NG = (NO * 3 + MV_NT - 1) / MV_NT;
+    int g_off[NG], g_xi[NG];
+    unsigned g_m[NG];
     for (int i = tid; i < NH; i += MV_NT) s_node[i] = halo[i];
     if (tid < NO) s_nm[tid] = A.nmask[(int64_t)blockIdx.x * NO + tid];
     const int ne = B.e_cnt;
@@ -266,7 +278,27 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             }
         }
         __syncthreads();
-        if (IN == IN_V && chunk + 2 < NCH) prefetch(reg, chunk + 2, s_node);
+        if (IN == IN_V && chunk + 2 < NCH) prefetch(reg, chunk + 2);
+        if (chunk == 0) {
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                const int i = tid + j * MV_NT;
+                int off = -1, xi = 0;
+                unsigned m = 0;
+                if (i < NO * 3) {
+                    const int o = i / 3, d = i % 3, h = ohalo(o);
+                    const int q = s_node[h];
+                    if (q >= 0) {
+                        off = ((q & NODE_MASK) * 3 + d) * NCH;
+                        xi = h * 3 + d;
+                        m = (unsigned)s_nm[o] | ((q & BC_BIT) ? 256u : 0u) | ((unsigned)o << 9) | ((unsigned)d << 16);
+                    }
+                }
+                g_off[j] = off;
+                g_xi[j] = xi;
+                g_m[j] = m;
+            }
+        }
 
         // ---- element products: task t = (tile element t / RC, load case t % RC)
         {
@@ -313,21 +345,22 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
         double acc[RC];
 #pragma unroll
         for (int w = 0; w < RC; ++w) acc[w] = 0.0;
-        for (int i = tid; i < NO * 3; i += MV_NT) {
-            const int o = i / 3, d = i % 3;
-            const int h = ohalo(o);
-            const int q = s_node[h];
-            if (q < 0) continue;
-            const unsigned m = s_nm[o];
+#pragma unroll
+        for (int j = 0; j < NG; ++j) {
+            if (g_off[j] < 0) continue;
+            const unsigned m = g_m[j];
+            const int od = (int)(m >> 9);          // o | d << 7
+            const int o = od & 127, d = od >> 7;
             V y;
             vzero(y);
 #pragma unroll
             for (int b = 0; b < 8; ++b)
                 if (m & (1u << b)) vadd(y, s_c[(OB_BIAS + b * NO + o) * 3 + d]);
-            const bool bc = (q & BC_BIT) != 0;
+            const bool bc = (m & 256u) != 0;
+            const int vi = g_off[j] + chunk;
             if (OUT == OUT_MASKED_DOT) {
                 if (bc) vzero(y);
-                const V x = s_x[h * 3 + d];
+                const V x = s_x[g_xi[j]];
 #pragma unroll
                 for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(x, w) * (double)vcomp(y, w);
             } else if (OUT == OUT_RESID) {
@@ -337,7 +370,6 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             } else if (OUT == OUT_MASKED) {
                 if (bc) vzero(y);
             } else if (OUT == OUT_BRES || OUT == OUT_CHEB) {
-                const int64_t vi = ((int64_t)(q & NODE_MASK) * 3 + d) * NCH + chunk;
                 const V bv = reinterpret_cast<const V*>(A.bvec)[vi];
                 if (OUT == OUT_BRES) {
                     if (bc) vzero(y);
@@ -352,8 +384,8 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                     vzero(dn);
                     vzero(xn);
                     if (!bc) {
-                        const T di = reinterpret_cast<const T*>(A.dinv)[q & NODE_MASK];
-                        const V dv = *dp, xv = s_x[h * 3 + d];
+                        const T di = reinterpret_cast<const T*>(A.dinv)[g_off[j] / (3 * NCH)];
+                        const V dv = *dp, xv = s_x[g_xi[j]];
                         T* dnp = reinterpret_cast<T*>(&dn);
                         T* xnp = reinterpret_cast<T*>(&xn);
 #pragma unroll
@@ -370,7 +402,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                         for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(bv, w) * (double)vcomp(y, w);
                 }
             }
-            yv[((int64_t)(q & NODE_MASK) * 3 + d) * NCH + chunk] = y;
+            yv[vi] = y;
         }
         if (OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) {
             // every lane holds the RC load cases of the chunk: fixed butterfly over the wave
@@ -1726,7 +1758,7 @@ struct LvView {
 // Chebyshev smoothing parameters: interval [lmax / ratio, lmax] of D^-1 K
 struct MgParams {
     double ratio = 10.0;       // smoothing passes of the fine and intermediate levels
-    int coarse_degree = 8;     // coarsest level
+    int coarse_degree = 4;     // coarsest level
     double coarse_ratio = 50.0;
 };
 
