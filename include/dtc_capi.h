@@ -125,8 +125,10 @@ typedef struct {
     int64_t iterations;         /* PCG iterations (inner, summed) */
     int64_t outer_iterations;   /* FP64 residual evaluations */
     double relres_max;          /* max over load cases of the true relative residual */
-    double ff[576];             /* "Domain forces" ff(dof, lc), column-major 24x24 */
-    double K_macro[576];        /* "Effective numerical stiffness" ff vv^-1 */
+    double ff[3600];            /* "Domain forces" ff(dof, lc), column-major R x R, R = 24 (HEX08
+                                   macro element) or 60 (HEX20), dof = (d-1)*nodes + l */
+    double K_macro[3600];       /* "Effective numerical stiffness" ff vv^-1, R x R (HEX20: vv from
+                                   the boundary table, the reference's own vv being singular) */
     double C[36];               /* "Effective stiffness" fv meps */
     double EE[36];              /* "Averaged Effective stiffness" (C + C^T)/2 */
     double density;             /* "Effective density" */

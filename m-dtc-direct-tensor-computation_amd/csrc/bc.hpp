@@ -7,8 +7,14 @@
 // (f-src/mod_write_deck.f90:63-107).  The table below lists the nonzero
 // entries of uu in the reference's order; the sum over l runs in increasing
 // l, so skipping the zero terms changes no bit of the result.
+//
+// The HEX20 macro element (60 load cases) uses the ANSYS 20-node shape
+// functions (f-src/mod_linfe.f90:827-888) and init_displ_hexe20
+// (f-src/mod_write_deck.f90:118-279, table in uu20_table.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "uu20_table.hpp"
 
 namespace dtc {
 
@@ -67,6 +73,64 @@ __host__ __device__ inline double bc_value(const double N[8], int d, int lc) {
     for (int k = 0; k < cnt; ++k)
         if (e[k].d == d) s += N[e[k].l] * ((double)e[k].m * 1.e-6);
     return s;
+}
+
+// N(xi(x)) for the HEX20 macro element in the reference's operation order
+// (products left to right; f = 1/8 and ff = 1/4 are exact in single precision)
+__host__ __device__ inline void macro_shape20(const double x[3], const double mn[3], const double mx[3],
+                                              double N[20]) {
+#pragma clang fp contract(off)
+    double xi[3];
+    for (int d = 0; d < 3; ++d) xi[d] = 2.0 / (mx[d] - mn[d]) * x[d] - (mx[d] + mn[d]) / (mx[d] - mn[d]);
+    const double f = 0.125, ff = 0.25, e = 1.0, z = 2.0;
+    const double a = xi[0], b = xi[1], c = xi[2];
+    N[0] = f * (e - a) * (e - b) * (e - c) * (-a - b - c - z);
+    N[1] = f * (e + a) * (e - b) * (e - c) * (a - b - c - z);
+    N[2] = f * (e + a) * (e + b) * (e - c) * (a + b - c - z);
+    N[3] = f * (e - a) * (e + b) * (e - c) * (-a + b - c - z);
+    N[4] = f * (e - a) * (e - b) * (e + c) * (-a - b + c - z);
+    N[5] = f * (e + a) * (e - b) * (e + c) * (a - b + c - z);
+    N[6] = f * (e + a) * (e + b) * (e + c) * (a + b + c - z);
+    N[7] = f * (e - a) * (e + b) * (e + c) * (-a + b + c - z);
+    N[8] = ff * (e - a * a) * (e - b) * (e - c);
+    N[9] = ff * (e + a) * (e - b * b) * (e - c);
+    N[10] = ff * (e - a * a) * (e + b) * (e - c);
+    N[11] = ff * (e - a) * (e - b * b) * (e - c);
+    N[12] = ff * (e - a * a) * (e - b) * (e + c);
+    N[13] = ff * (e + a) * (e - b * b) * (e + c);
+    N[14] = ff * (e - a * a) * (e + b) * (e + c);
+    N[15] = ff * (e - a) * (e - b * b) * (e + c);
+    N[16] = ff * (e - a) * (e - b) * (e - c * c);
+    N[17] = ff * (e + a) * (e - b) * (e - c * c);
+    N[18] = ff * (e + a) * (e + b) * (e - c * c);
+    N[19] = ff * (e - a) * (e + b) * (e - c * c);
+}
+
+// HEX20 macro boundary value of direction d, load case lc (0..59)
+__host__ __device__ inline double bc_value20(const double N[20], int d, int lc) {
+#pragma clang fp contract(off)
+    int cnt;
+    const UU20Entry* e = uu20_entries(lc, &cnt);
+    double s = 0.0;
+    for (int k = 0; k < cnt; ++k)
+        if (e[k].d == d) s += N[e[k].l] * (e[k].m * 1.e-6);
+    return s;
+}
+
+// load cases and macro nodes of a macro element order (1: HEX08, 2: HEX20)
+__host__ __device__ constexpr int macro_lcs(int order) { return order == 2 ? 60 : 24; }
+__host__ __device__ constexpr int macro_nodes(int order) { return order == 2 ? 20 : 8; }
+
+__host__ __device__ inline void macro_shape(int order, const double x[3], const double mn[3], const double mx[3],
+                                            double N[20]) {
+    if (order == 2) macro_shape20(x, mn, mx, N);
+    else macro_shape8(x, mn, mx, N);
+}
+
+// boundary value of global load case lc; 0 beyond the element's load cases
+__host__ __device__ inline double bc_value_m(int order, const double N[20], int d, int lc) {
+    if (lc >= macro_lcs(order)) return 0.0;
+    return order == 2 ? bc_value20(N, d, lc) : bc_value(N, d, lc);
 }
 
 // face test of generate_boundaries (f-src/mod_struct_preprocess.f90:564-592)

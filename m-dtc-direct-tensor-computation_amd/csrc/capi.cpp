@@ -6,6 +6,7 @@
 // EE = (C + C^T)/2.  Everything proportional to the mesh size runs on the GPU.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -69,6 +70,19 @@ void vv_lin(double rve_strain, double vv[576]) {
     }
 }
 
+// HEX20 macro load-case matrix vv(dof, lc) = uu(l, d, lc) * RVE_STRAIN / 1e-6, dof = d*20 + l.
+// The reference's own table (init_loadcase_el_order_quad, f-src/mod_struct_calcmat.f90:
+// 1780-2028) leaves row 49 unassigned and holds unscaled literals, so it is
+// singular; K_M of the HEX20 macro element is formed with this matrix instead.
+void vv_quad(double rve_strain, double* vv) {
+    std::memset(vv, 0, sizeof(double) * 3600);
+    for (int lc = 0; lc < 60; ++lc) {
+        int cnt;
+        const UU20Entry* e = uu20_entries(lc, &cnt);
+        for (int k = 0; k < cnt; ++k) vv[(e[k].d * 20 + e[k].l) + 60 * lc] = e[k].m * rve_strain;
+    }
+}
+
 int invert(double* A, int n) {
     std::vector<double> M(A, A + n * n), I(n * n, 0.0);
     for (int i = 0; i < n; ++i) I[i + n * i] = 1.0;
@@ -111,24 +125,44 @@ void fv_weights(const double xdp[3], double W[6 * 24]) {
     set(5, {19, 20, 23, 24}, 0.5 / A13);
 }
 
+// the same for the HEX20 macro element (f-src/mod_struct_calcmat.f90:365-401): 6 x 60
+void fv_weights_quad(const double xdp[3], double W[6 * 60]) {
+    std::memset(W, 0, sizeof(double) * 6 * 60);
+    const double A23 = xdp[1] * xdp[2], A13 = xdp[0] * xdp[2], A12 = xdp[0] * xdp[1];
+    auto set = [&](int i, std::initializer_list<int> rows, double w) { for (int r : rows) W[i + 6 * (r - 1)] += w; };
+    set(0, {2, 3, 6, 7, 10, 14, 18, 19}, 1.0 / A23);
+    set(1, {23, 24, 27, 28, 31, 35, 39, 40}, 1.0 / A13);
+    set(2, {45, 46, 47, 48, 53, 54, 55, 56}, 1.0 / A12);
+    set(3, {3, 4, 7, 8, 11, 15, 19, 20}, 0.5 / A13);
+    set(3, {22, 23, 26, 27, 30, 34, 38, 39}, 0.5 / A23);
+    set(4, {5, 6, 7, 8, 13, 14, 15, 16}, 0.5 / A12);
+    set(4, {42, 43, 46, 47, 50, 54, 58, 59}, 0.5 / A23);
+    set(5, {25, 26, 27, 28, 33, 34, 35, 36}, 0.5 / A12);
+    set(5, {43, 44, 47, 48, 51, 55, 59, 60}, 0.5 / A13);
+}
+
 constexpr double MEPS[6] = {1.e6, 1.e6, 1.e6, -1.e6, -1.e6, 1.e6};   // :434-441
 
-int effective(const double* ff, const double xdp[3], double rve_strain, double* K_M, double* C, double* EE) {
-    double vv[576];
-    vv_lin(rve_strain, vv);
-    DTC_TRY(invert(vv, R));
-    for (int i = 0; i < R; ++i)
-        for (int j = 0; j < R; ++j) {
+// ff, K_M: column-major RM x RM, RM = 24 (macro order 1) or 60 (order 2)
+int effective(const double* ff, int order, const double xdp[3], double rve_strain, double* K_M, double* C, double* EE) {
+    const int RM = macro_lcs(order);
+    std::vector<double> vv(3600);
+    if (order == 1) vv_lin(rve_strain, vv.data());
+    else vv_quad(rve_strain, vv.data());
+    DTC_TRY(invert(vv.data(), RM));
+    for (int i = 0; i < RM; ++i)
+        for (int j = 0; j < RM; ++j) {
             double s = 0.0;
-            for (int k = 0; k < R; ++k) s += ff[i + R * k] * vv[k + R * j];
-            K_M[i + R * j] = s;
+            for (int k = 0; k < RM; ++k) s += ff[i + RM * k] * vv[k + RM * j];
+            K_M[i + RM * j] = s;
         }
-    double W[144], fv[36];
-    fv_weights(xdp, W);
+    double W[6 * 60], fv[36];
+    if (order == 1) fv_weights(xdp, W);
+    else fv_weights_quad(xdp, W);
     for (int i = 0; i < 6; ++i)
         for (int j = 0; j < 6; ++j) {
             double s = 0.0;
-            for (int a = 0; a < R; ++a) s += W[i + 6 * a] * ff[a + R * j];
+            for (int a = 0; a < RM; ++a) s += W[i + 6 * a] * ff[a + RM * j];
             fv[i + 6 * j] = s;
         }
     for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) C[i + 6 * j] = fv[i + 6 * j] * MEPS[j];
@@ -204,7 +238,7 @@ int validate_desc(const dtc_domain_desc& d) {
         if (!(d.delta[k] > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "grid spacing delta[%d] = %g", k, d.delta[k]);
     }
     if (d.micro_nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "the solve implements HEX08 micro elements (got %d nodes)", d.micro_nen);
-    if (d.macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "the solve implements the HEX08 macro element (order %d)", d.macro_order);
+    if (d.macro_order != 1 && d.macro_order != 2) return set_error(DTC_ERR_BAD_ARGUMENT, "macro element order %d", d.macro_order);
     if (!(d.nu > -1.0 && d.nu < 0.5) || !(d.E > 0.0) || !(d.size_domain > 0.0))
         return set_error(DTC_ERR_BAD_ARGUMENT, "material E=%g nu=%g size_domain=%g", d.E, d.nu, d.size_domain);
     return DTC_OK;
@@ -222,6 +256,7 @@ struct dtc_batch {
     Stream st;
     std::vector<std::unique_ptr<DevMem>> phi, mod;
     std::vector<double> escale;
+    std::vector<int> g0, ng;       // first solver group and group count of each domain (24 load cases per group)
     SolverBatch sb;
     MeshDev mesh;
     std::vector<double> ff, EE;
@@ -345,7 +380,8 @@ int dtc_mesh_build(const void* phi, int kind, const int64_t n[3], int64_t llimit
 int dtc_boundaries(const double* nodes, int64_t n_nodes, const double min_c[3], const double max_c[3],
                    const double dim_c[3], int macro_order, int64_t* n_b, int64_t* bnode_ids, double* bvals) {
     if (!nodes || !min_c || !max_c || !dim_c || !n_b || n_nodes < 0) return set_error(DTC_ERR_BAD_ARGUMENT, "bad argument");
-    if (macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "boundary tables implement the HEX08 macro element");
+    if (macro_order != 1 && macro_order != 2) return set_error(DTC_ERR_BAD_ARGUMENT, "macro order %d", macro_order);
+    const int RM = macro_lcs(macro_order);
     if ((bnode_ids == nullptr) != (bvals == nullptr)) return set_error(DTC_ERR_BAD_ARGUMENT, "ids and values go together");
     DTC_TRY(check_device());
     Stream st;
@@ -354,14 +390,15 @@ int dtc_boundaries(const double* nodes, int64_t n_nodes, const double min_c[3], 
     DTC_TRY(dn.alloc((size_t)n_nodes * 3 * 8));
     DTC_HIP(hipMemcpyAsync(dn.p, nodes, (size_t)n_nodes * 3 * 8, hipMemcpyHostToDevice, st.s));
     int64_t nb = 0;
-    DTC_TRY(boundaries_device(dn.as<double>(), n_nodes, min_c, max_c, dim_c, &nb, nullptr, nullptr, flag, pos, tmp, st.s));
+    DTC_TRY(boundaries_device(dn.as<double>(), n_nodes, min_c, max_c, dim_c, macro_order, &nb, nullptr, nullptr, flag,
+                              pos, tmp, st.s));
     if (bnode_ids) {
         DTC_TRY(di.alloc((size_t)nb * 8));
-        DTC_TRY(dv.alloc((size_t)nb * 3 * R * 8));
-        DTC_TRY(boundaries_device(dn.as<double>(), n_nodes, min_c, max_c, dim_c, &nb, di.as<int64_t>(), dv.as<double>(), flag,
-                                  pos, tmp, st.s));
+        DTC_TRY(dv.alloc((size_t)nb * 3 * RM * 8));
+        DTC_TRY(boundaries_device(dn.as<double>(), n_nodes, min_c, max_c, dim_c, macro_order, &nb, di.as<int64_t>(),
+                                  dv.as<double>(), flag, pos, tmp, st.s));
         DTC_HIP(hipMemcpyAsync(bnode_ids, di.p, (size_t)nb * 8, hipMemcpyDeviceToHost, st.s));
-        DTC_HIP(hipMemcpyAsync(bvals, dv.p, (size_t)nb * 3 * R * 8, hipMemcpyDeviceToHost, st.s));
+        DTC_HIP(hipMemcpyAsync(bvals, dv.p, (size_t)nb * 3 * RM * 8, hipMemcpyDeviceToHost, st.s));
         DTC_HIP(hipStreamSynchronize(st.s));
     }
     *n_b = nb;
@@ -423,16 +460,28 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
     DTC_HIP(hipStreamSynchronize(s));
     const auto t0 = std::chrono::steady_clock::now();
     b->sb.timing = true;
-    DTC_TRY(batch_begin(b->sb, b->d.data(), D));
+    // each domain is solved as groups of 24 macro load cases (1 group for the
+    // HEX08 macro element, 3 for the 60 load cases of the HEX20 one)
+    std::vector<dtc_domain_desc> gd;
+    b->g0.assign(D, 0);
+    b->ng.assign(D, 0);
+    for (int i = 0; i < D; ++i) {
+        b->g0[i] = (int)gd.size();
+        b->ng[i] = (macro_lcs(b->d[i].macro_order) + R - 1) / R;
+        for (int g = 0; g < b->ng[i]; ++g) gd.push_back(b->d[i]);
+    }
+    DTC_TRY(batch_begin(b->sb, gd.data(), (int64_t)gd.size()));
     for (int i = 0; i < D; ++i) {
         const dtc_domain_desc& d = b->d[i];
         int rc = mesh_build_device(b->phi[i]->p, d.phi_kind, d.n, d.llimit, d.micro_nen, b->mesh, s);
         if (rc == DTC_ERR_EMPTY_MESH) {
-            DTC_TRY(batch_add_empty(b->sb, d, DTC_ERR_EMPTY_MESH));
+            for (int g = 0; g < b->ng[i]; ++g) DTC_TRY(batch_add_empty(b->sb, d, DTC_ERR_EMPTY_MESH, R * g));
             continue;
         }
         DTC_TRY(rc);
-        DTC_TRY(batch_add_domain(b->sb, b->mesh, d, b->mod[i]->p ? b->mod[i]->as<double>() : nullptr, b->escale[i], s));
+        for (int g = 0; g < b->ng[i]; ++g)
+            DTC_TRY(batch_add_domain(b->sb, b->mesh, d, b->mod[i]->p ? b->mod[i]->as<double>() : nullptr, b->escale[i],
+                                     R * g, s));
     }
     DTC_TRY(batch_finalize(b->sb, b->o.precision));
     b->sb.precond = (b->o.precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG) ? DTC_PC_MG : DTC_PC_JACOBI;
@@ -443,17 +492,24 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
     b->EE.assign((size_t)D * 36, 0.0);
     std::vector<dtc_domain_result> out(D);
     std::vector<int> solved;
+    constexpr int FFS = 60;   // macro DOF slots per load case in the force partials
     for (int i = 0; i < D; ++i) {
         dtc_domain_result& r = out[i];
         std::memset(&r, 0, sizeof(r));
         const dtc_domain_desc& d = b->d[i];
-        r.status = b->sb.h_status[i];
-        r.n_elems = b->sb.h_elems[i];
-        r.n_nodes = b->sb.h_nodes[i];
-        r.n_bnodes = b->sb.h_bnodes[i];
-        r.iterations = b->iters[i];
-        r.outer_iterations = b->outer[i];
-        r.relres_max = b->relres[i];
+        const int gi = b->g0[i], RM = macro_lcs(d.macro_order), FD = 3 * macro_nodes(d.macro_order);
+        r.status = b->sb.h_status[gi];
+        for (int g = 0; g < b->ng[i]; ++g) {
+            const int st = b->sb.h_status[gi + g];
+            if (r.status == DTC_OK && st != DTC_OK) r.status = st;
+            r.iterations = std::max<int64_t>(r.iterations, b->iters[gi + g]);
+            r.outer_iterations = std::max<int64_t>(r.outer_iterations, b->outer[gi + g]);
+            const double rr = b->relres[gi + g];
+            r.relres_max = (std::isnan(rr) || std::isnan(r.relres_max)) ? NAN : std::max(r.relres_max, rr);
+        }
+        r.n_elems = b->sb.h_elems[gi];
+        r.n_nodes = b->sb.h_nodes[gi];
+        r.n_bnodes = b->sb.h_bnodes[gi];
         double xdp[3], v = 1.0;
         for (int k = 0; k < 3; ++k) {
             xdp[k] = (double)d.n[k] * d.delta[k];
@@ -461,8 +517,12 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         }
         r.density = (double)r.n_elems / v;   // f-src/mod_struct_calcmat.f90:1642-1646
         if (r.status != DTC_OK && r.status != DTC_ERR_NO_CONVERGENCE) continue;
-        std::memcpy(r.ff, &b->ff[(size_t)i * 576], sizeof(r.ff));
-        DTC_TRY(effective(r.ff, xdp, d.rve_strain, r.K_macro, r.C, r.EE));
+        // ff(dof, lc), column-major RM x RM
+        for (int g = 0; g < b->ng[i]; ++g)
+            for (int l = 0; l < R && R * g + l < RM; ++l)
+                for (int k = 0; k < FD; ++k)
+                    r.ff[k + RM * (R * g + l)] = b->ff[((size_t)(gi + g) * R + l) * FFS + k];
+        DTC_TRY(effective(r.ff, d.macro_order, xdp, d.rve_strain, r.K_macro, r.C, r.EE));
         std::memcpy(&b->EE[(size_t)i * 36], r.EE, sizeof(r.EE));
         solved.push_back(i);
     }
@@ -496,8 +556,10 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
     if (!b->ran) return set_error(DTC_ERR_BAD_ARGUMENT, "dtc_batch_grad before dtc_batch_run");
     if (i < 0 || i >= (int64_t)b->d.size()) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %lld", (long long)i);
     const dtc_domain_desc& d = b->d[i];
+    if (d.macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements the HEX08 macro element");
+    const int gi = b->g0[i];
     const size_t nv = (size_t)d.n[0] * d.n[1] * d.n[2];
-    if (b->sb.h_status[i] != DTC_OK && b->sb.h_status[i] != DTC_ERR_NO_CONVERGENCE) {
+    if (b->sb.h_status[gi] != DTC_OK && b->sb.h_status[gi] != DTC_ERR_NO_CONVERGENCE) {
         std::memset(dL_dE, 0, nv * 8);
         return DTC_OK;
     }
@@ -519,7 +581,7 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
             M[bb * 6 + j] = s * MEPS[j];
         }
     std::vector<double> g(nv);
-    DTC_TRY(batch_grad(b->sb, (int)i, M, g.data()));
+    DTC_TRY(batch_grad(b->sb, gi, M, g.data()));
     const double sc = b->escale[i];   // d(emod)/d(E_v)
     for (size_t v = 0; v < nv; ++v) dL_dE[v] = g[v] * sc;
     return DTC_OK;
@@ -566,8 +628,13 @@ int dtc_solve_domain(const dtc_domain_desc* d, const dtc_solve_opts* o, dtc_doma
     dtc_batch* b = nullptr;
     DTC_TRY(dtc_batch_create(d, 1, o, &b));
     int rc = dtc_batch_run(b, res);
-    if (rc == DTC_OK && (u_out || r_out) && (res->status == DTC_OK || res->status == DTC_ERR_NO_CONVERGENCE))
-        rc = batch_fields(b->sb, 0, u_out, r_out);
+    if (rc == DTC_OK && (u_out || r_out) && (res->status == DTC_OK || res->status == DTC_ERR_NO_CONVERGENCE)) {
+        const int RM = macro_lcs(d->macro_order);
+        const size_t col = (size_t)3 * res->n_nodes;
+        for (int g = 0; g < b->ng[0] && rc == DTC_OK; ++g)
+            rc = batch_fields(b->sb, b->g0[0] + g, std::min(R, RM - R * g), u_out ? u_out + col * R * g : nullptr,
+                              r_out ? r_out + col * R * g : nullptr);
+    }
     dtc_batch_destroy(b);
     return rc;
 }

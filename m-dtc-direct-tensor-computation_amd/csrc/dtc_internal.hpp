@@ -124,6 +124,8 @@ int scan_exclusive_i32(const int* in, int* out, int64_t n, int64_t* total, DevMe
 // ---------------------------------------------------------------- solver topology
 struct DomInfo {
     int nx, ny, nz;           // voxels per edge
+    int macro_order;          // 1: HEX08 macro element (24 load cases), 2: HEX20 (60)
+    int lc_base;              // first macro load case of this group of 24
     int bnx, bny, bnz;        // brick grid
     int brick_begin, brick_end;
     int64_t lat_off;          // offset into the lattice map

@@ -5,8 +5,9 @@
 //  * k_bc_flag / k_bc_write: generate_boundaries (f-src/mod_struct_preprocess.f90:
 //    490-738): boundary nodes in node order, tested against the domain box
 //    faces in the reference's order, and their prescribed displacements of
-//    the 24 load cases (f-src/mod_write_deck.f90:22-107), bit-identical to
-//    the reference's FP64 operations.
+//    the 24 (HEX08 macro) or 60 (HEX20 macro) load cases
+//    (f-src/mod_write_deck.f90:22-279), bit-identical to the reference's FP64
+//    operations.
 #include "bc.hpp"
 #include "dtc_internal.hpp"
 #include "io.hpp"
@@ -36,16 +37,17 @@ __global__ void k_bc_flag(const double* __restrict__ nodes, int64_t n, double mn
 
 __global__ void k_bc_write(const double* __restrict__ nodes, int64_t n, const int* __restrict__ flag,
                            const int* __restrict__ pos, double mn0, double mn1, double mn2, double mx0, double mx1,
-                           double mx2, int64_t* __restrict__ ids, double* __restrict__ vals) {
+                           double mx2, int order, int64_t* __restrict__ ids, double* __restrict__ vals) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !flag[i]) return;
     const double mn[3] = {mn0, mn1, mn2}, mx[3] = {mx0, mx1, mx2};
     const int64_t b = pos[i];
+    const int RM = macro_lcs(order);
     ids[b] = i + 1;
-    double N[8];
-    macro_shape8(nodes + 3 * i, mn, mx, N);
+    double N[20];
+    macro_shape(order, nodes + 3 * i, mn, mx, N);
     for (int d = 0; d < 3; ++d)
-        for (int lc = 0; lc < R24; ++lc) vals[lc + (int64_t)R24 * (3 * b + d)] = bc_value(N, d, lc);
+        for (int lc = 0; lc < RM; ++lc) vals[lc + (int64_t)RM * (3 * b + d)] = bc_value_m(order, N, d, lc);
 }
 
 }  // namespace
@@ -68,8 +70,8 @@ int extract_block_device(const void* vol_dev, int kind, const int64_t vdim[3], c
 }
 
 int boundaries_device(const double* nodes_dev, int64_t n, const double mn[3], const double mx[3], const double dim[3],
-                      int64_t* n_b, int64_t* ids_dev, double* vals_dev, DevMem& flag, DevMem& pos, DevMem& tmp,
-                      hipStream_t s) {
+                      int order, int64_t* n_b, int64_t* ids_dev, double* vals_dev, DevMem& flag, DevMem& pos,
+                      DevMem& tmp, hipStream_t s) {
     if (n <= 0) { *n_b = 0; return DTC_OK; }
     DTC_TRY(flag.alloc((size_t)n * 4));
     DTC_TRY(pos.alloc((size_t)n * 4));
@@ -79,7 +81,7 @@ int boundaries_device(const double* nodes_dev, int64_t n, const double mn[3], co
     DTC_TRY(scan_exclusive_i32(flag.as<int>(), pos.as<int>(), n, n_b, tmp, s));
     if (ids_dev && vals_dev) {
         k_bc_write<<<g, NT, 0, s>>>(nodes_dev, n, flag.as<int>(), pos.as<int>(), mn[0], mn[1], mn[2], mx[0], mx[1], mx[2],
-                                    ids_dev, vals_dev);
+                                    order, ids_dev, vals_dev);
         DTC_HIP(hipGetLastError());
     }
     return DTC_OK;

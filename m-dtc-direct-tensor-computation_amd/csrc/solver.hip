@@ -266,10 +266,10 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                 if (q >= 0) {
                     if (q & BC_BIT) {
                         const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
-                        double c[3], N[8];
+                        double c[3], N[20];
                         node_coord(D, B.x0 - 1 + hx, B.y0 - 1 + hy, B.z0 - 1 + hz, c);
-                        macro_shape8(c, D.min_c, D.max_c, N);
-                        v = (T)bc_value(N, d, lc);
+                        macro_shape(D.macro_order, c, D.min_c, D.max_c, N);
+                        v = (T)bc_value_m(D.macro_order, N, d, D.lc_base + lc);
                     } else if (xin) {
                         v = xin[((int64_t)(q & NODE_MASK) * 3 + d) * R + lc];
                     }
@@ -872,61 +872,75 @@ __global__ void k_brick_tables(const BrickInfo* __restrict__ bricks, const DomIn
 }
 
 // ---------------------------------------------------------------- force projection
-// ff(d*8+l, lc) = sum over owned nodes r_d(node, lc) N_l(xi(node))
-// (f-src/mod_struct_calcmat.f90:254-280); partial per brick [brick][lc][24]
+// ff(d*NEN + l, lc) = sum over owned nodes r_d(node, lc) N_l(xi(node))
+// (f-src/mod_struct_calcmat.f90:254-280) with the NEN macro shape functions;
+// partial per brick [brick][24 lc][FFD]; bricks of the other macro order are skipped
+constexpr int FFD = 60;   // macro DOF slots of the force partials (3 x 20)
+
+template <int NEN>
 __global__ __launch_bounds__(NT_UPD) void k_project(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
                                                     const int* __restrict__ lat2node, const double* __restrict__ rf,
                                                     double* __restrict__ pff) {
     constexpr int G = NT_UPD / R;    // 8 node groups
-    __shared__ double s[G][R][24];
+    constexpr int FD = 3 * NEN, PASS = 12;
+    __shared__ double s[G][R][PASS];
     const BrickInfo B = bricks[blockIdx.x];
     const DomInfo D = doms[B.dom];
+    if (macro_nodes(D.macro_order) != NEN) return;
     const int tid = threadIdx.x, lc = tid % R, g = tid / R;
     const int Lx = D.nx + 1, Ly = D.ny + 1;
-    double acc[24];
+    double acc[FD];
 #pragma unroll
-    for (int k = 0; k < 24; ++k) acc[k] = 0.0;
+    for (int k = 0; k < FD; ++k) acc[k] = 0.0;
     for (int o = g; o < BX * BY * BZ; o += G) {
         int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
         if (gx > D.nx || gy > D.ny || gz > D.nz) continue;
         int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
         if (q < 0) continue;
-        double c[3], N[8];
+        double c[3], N[20];
         node_coord(D, gx, gy, gz, c);
-        macro_shape8(c, D.min_c, D.max_c, N);
+        macro_shape(D.macro_order, c, D.min_c, D.max_c, N);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             double rv = rf[((int64_t)q * 3 + d) * R + lc];
 #pragma unroll
-            for (int l = 0; l < 8; ++l) acc[d * 8 + l] += rv * N[l];
+            for (int l = 0; l < NEN; ++l) acc[d * NEN + l] += rv * N[l];
         }
     }
 #pragma unroll
-    for (int k = 0; k < 24; ++k) s[g][lc][k] = acc[k];
-    __syncthreads();
-    for (int i = tid; i < R * 24; i += NT_UPD) {
-        int l2 = i / 24, k = i % 24;
-        double v = 0.0;
-        for (int gg = 0; gg < G; ++gg) v += s[gg][l2][k];
-        pff[(int64_t)blockIdx.x * R * 24 + i] = v;
+    for (int k0 = 0; k0 < FD; k0 += PASS) {
+#pragma unroll
+        for (int k = 0; k < PASS; ++k) s[g][lc][k] = k0 + k < FD ? acc[k0 + k] : 0.0;
+        __syncthreads();
+        for (int i = tid; i < R * PASS; i += NT_UPD) {
+            const int l2 = i / PASS, k = i % PASS;
+            if (k0 + k >= FD) continue;
+            double v = 0.0;
+            for (int gg = 0; gg < G; ++gg) v += s[gg][l2][k];
+            pff[((int64_t)blockIdx.x * R + l2) * FFD + k0 + k] = v;
+        }
+        __syncthreads();
     }
 }
 
 __global__ void k_red_ff(const DomInfo* __restrict__ doms, const double* __restrict__ pff, double* __restrict__ ff) {
     const int d = blockIdx.x;
     const DomInfo D = doms[d];
-    for (int i = threadIdx.x; i < R * 24; i += blockDim.x) {
+    const int FD = 3 * macro_nodes(D.macro_order);
+    for (int i = threadIdx.x; i < R * FFD; i += blockDim.x) {
         double v = 0.0;
-        for (int b = D.brick_begin; b < D.brick_end; ++b) v += pff[(int64_t)b * R * 24 + i];
-        ff[(int64_t)d * R * 24 + i] = v;
+        if (i % FFD < FD)
+            for (int b = D.brick_begin; b < D.brick_end; ++b) v += pff[(int64_t)b * R * FFD + i];
+        ff[(int64_t)d * R * FFD + i] = v;
     }
 }
 
-// u (mesh numbering) = x + x_bc, column-major (3N, 24)
+// u (mesh numbering) = x + x_bc, column-major (3N, ncol): load cases lc_base ..
+// lc_base + ncol - 1 of the domain's group
 __global__ __launch_bounds__(NT_UPD) void k_fields(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
                                                    const int* __restrict__ lat2node, const int* __restrict__ sol2ref,
                                                    const double* __restrict__ x, const double* __restrict__ rf,
-                                                   int dom, int64_t nn, double* __restrict__ uo,
+                                                   int dom, int64_t nn, int ncol, double* __restrict__ uo,
                                                    double* __restrict__ ro) {
     const BrickInfo B = bricks[blockIdx.x];
     if (B.dom != dom) return;
@@ -934,6 +948,7 @@ __global__ __launch_bounds__(NT_UPD) void k_fields(const BrickInfo* __restrict__
     const int Lx = D.nx + 1, Ly = D.ny + 1;
     for (int i = threadIdx.x; i < BX * BY * BZ * 3 * R; i += NT_UPD) {
         int o = i / (3 * R), rem = i % (3 * R), d = rem / R, lc = rem % R;
+        if (lc >= ncol) continue;
         int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
         if (gx > D.nx || gy > D.ny || gz > D.nz) continue;
         int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
@@ -942,10 +957,10 @@ __global__ __launch_bounds__(NT_UPD) void k_fields(const BrickInfo* __restrict__
         int64_t out = (3 * ref + d) + 3 * nn * lc;
         double v = x[((int64_t)q * 3 + d) * R + lc];
         if (is_bc(D, gx, gy, gz)) {
-            double c[3], N[8];
+            double c[3], N[20];
             node_coord(D, gx, gy, gz, c);
-            macro_shape8(c, D.min_c, D.max_c, N);
-            v = bc_value(N, d, lc);
+            macro_shape(D.macro_order, c, D.min_c, D.max_c, N);
+            v = bc_value_m(D.macro_order, N, d, D.lc_base + lc);
         }
         if (uo) uo[out] = v;
         if (ro) ro[out] = rf[((int64_t)q * 3 + d) * R + lc];
@@ -1344,9 +1359,11 @@ int batch_begin(SolverBatch& b, const dtc_domain_desc* d, int64_t n_dom) {
 }
 
 // Empty or failed domains keep a DomInfo with no bricks so indices stay aligned.
-int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status) {
+int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status, int lc_base) {
     DomInfo D{};
     D.nx = (int)d.n[0]; D.ny = (int)d.n[1]; D.nz = (int)d.n[2];
+    D.macro_order = d.macro_order;
+    D.lc_base = lc_base;
     D.brick_begin = D.brick_end = (int)b.h_brick.size();
     D.node_begin = D.node_end = b.n_nodes;
     D.lat_off = b.n_lat; D.vox_off = b.n_vox;
@@ -1362,10 +1379,12 @@ int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status) {
 }
 
 int batch_add_domain(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d, const double* modulus_dev,
-                     double escale, hipStream_t s) {
+                     double escale, int lc_base, hipStream_t s) {
     if (m.nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "the solver implements HEX08 micro elements");
     DomInfo D{};
     D.nx = m.nx; D.ny = m.ny; D.nz = m.nz;
+    D.macro_order = d.macro_order;
+    D.lc_base = lc_base;
     const int Lx = m.Lx, Ly = m.Ly, Lz = m.Lz;
     D.bnx = (Lx + BX - 1) / BX; D.bny = (Ly + BY - 1) / BY; D.bnz = (Lz + BZ - 1) / BZ;
     const int nbr = D.bnx * D.bny * D.bnz;
@@ -1386,7 +1405,7 @@ int batch_add_domain(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d,
         DTC_HIP(hipMemcpyAsync(&hb, nbc.p, 4, hipMemcpyDeviceToHost, s));
         DTC_HIP(hipStreamSynchronize(s));
         if (3 * hb < 6) {
-            DTC_TRY(batch_add_empty(b, d, DTC_ERR_FEW_CONSTRAINTS));
+            DTC_TRY(batch_add_empty(b, d, DTC_ERR_FEW_CONSTRAINTS, lc_base));
             b.h_elems.back() = m.n_elems;
             b.h_nodes.back() = m.n_nodes;
             b.h_bnodes.back() = hb;
@@ -1537,7 +1556,7 @@ int batch_finalize(SolverBatch& b, int precision) {
     DTC_TRY(b.part_a.ensure(nbk * R * 8));
     DTC_TRY(b.part_b.ensure(nbk * R * 8));
     DTC_TRY(b.part_c.ensure(nbk * R * 8));
-    DTC_TRY(b.part_ff.ensure(nbk * R * 24 * 8));
+    DTC_TRY(b.part_ff.ensure(nbk * R * FFD * 8));
     const size_t nd = (size_t)(b.n_dom > 0 ? b.n_dom : 1);
     for (DevMem* m : {&b.alpha, &b.beta, &b.rz, &b.rr, &b.rr0, &b.thr2, &b.scale}) DTC_TRY(m->alloc(nd * R * 8));
     DTC_TRY(b.active.ensure(nd * R * 4));
@@ -1546,7 +1565,7 @@ int batch_finalize(SolverBatch& b, int precision) {
     DTC_TRY(b.done.ensure(nd * 4));
     DTC_TRY(b.done64.ensure(nd * 4));
     DTC_TRY(b.iters.ensure(nd * 8));
-    DTC_TRY(b.ff.ensure(nd * R * 24 * 8));
+    DTC_TRY(b.ff.ensure(nd * R * FFD * 8));
     // element stiffness (E = 1) and its Walsh blocks
     double Khat[576];
     DTC_TRY(element_stiffness(8, 1.0, b.nu, b.a, Khat));
@@ -2052,13 +2071,15 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
 
 int batch_forces(SolverBatch& b, std::vector<double>& ff) {
     const int D = b.n_dom;
-    ff.assign((size_t)D * R * 24, 0.0);
+    ff.assign((size_t)D * R * FFD, 0.0);
     if (D == 0 || b.n_bricks == 0) return DTC_OK;
     const WK<double> wk64 = cast_wk<double>(b.wk64);
     // reactions on the unmodified operator: rf = K (x + x_bc), stored in r64
     DTC_TRY((launch_mv<double, IN_U, OUT_PLAIN>(b, wk64, b.x64.as<double>(), b.r64.as<double>(), nullptr, nullptr)));
-    k_project<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
-                                                   b.r64.as<double>(), b.part_ff.as<double>());
+    k_project<8><<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
+                                                      b.lat2node.as<int>(), b.r64.as<double>(), b.part_ff.as<double>());
+    k_project<20><<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
+                                                       b.lat2node.as<int>(), b.r64.as<double>(), b.part_ff.as<double>());
     k_red_ff<<<D, 256, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_ff.as<double>(), b.ff.as<double>());
     DTC_HIP(hipGetLastError());
     DTC_HIP(hipMemcpyAsync(ff.data(), b.ff.p, ff.size() * 8, hipMemcpyDeviceToHost, b.stream));
@@ -2067,17 +2088,18 @@ int batch_forces(SolverBatch& b, std::vector<double>& ff) {
     return DTC_OK;
 }
 
-int batch_fields(SolverBatch& b, int i, double* u_out, double* r_out) {
+int batch_fields(SolverBatch& b, int i, int ncol, double* u_out, double* r_out) {
     if (i < 0 || i >= b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %d", i);
     const int64_t nn = b.h_nodes[i];
     if (nn == 0) return DTC_OK;
     if (r_out && !b.reactions_valid) return set_error(DTC_ERR_BAD_ARGUMENT, "reactions requested before the force pass");
     DevMem du, dr;
-    const size_t bytes = (size_t)nn * 3 * R * 8;
+    if (ncol < 1 || ncol > R) return set_error(DTC_ERR_BAD_ARGUMENT, "field columns %d", ncol);
+    const size_t bytes = (size_t)nn * 3 * ncol * 8;
     if (u_out) DTC_TRY(du.alloc(bytes));
     if (r_out) DTC_TRY(dr.alloc(bytes));
     k_fields<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
-                                                  b.sol2ref.as<int>(), b.x64.as<double>(), b.r64.as<double>(), i, nn,
+                                                  b.sol2ref.as<int>(), b.x64.as<double>(), b.r64.as<double>(), i, nn, ncol,
                                                   u_out ? du.as<double>() : nullptr, r_out ? dr.as<double>() : nullptr);
     DTC_HIP(hipGetLastError());
     if (u_out) DTC_HIP(hipMemcpyAsync(u_out, du.p, bytes, hipMemcpyDeviceToHost, b.stream));

@@ -41,7 +41,7 @@ struct SolverBatch {
     DevMem x32, r32, p32, q32;
     // per-brick partial sums
     DevMem part_a, part_b, part_c; // [brick][24]
-    DevMem part_ff;                // [brick][24 lc][24]
+    DevMem part_ff;                // [brick][24 lc][60 macro DOF slots]
     // per-domain scalars [dom][24]
     DevMem alpha, beta, rz, rr, rr0, thr2, scale;
     DevMem active, active64;       // int [dom][24]
@@ -49,7 +49,7 @@ struct SolverBatch {
     DevMem done;                   // int [dom]: inner iteration finished
     DevMem done64;                 // int [dom]: outer (FP64 residual) converged
     DevMem iters;                  // int64 [dom]
-    DevMem ff;                     // double [dom][24 lc][24]
+    DevMem ff;                     // double [dom][24 lc][60 macro DOF slots]
     DevMem gtmp;                   // gradient work
     // multigrid preconditioner of the FP32 PCG (empty when Jacobi is used)
     std::vector<Level> mg;
@@ -78,13 +78,14 @@ int element_stiffness(int nen, double E, double nu, double a, double* K);
 
 // Reserve the lattice map and voxel modulus arrays of n_dom domains.
 int batch_begin(SolverBatch& b, const dtc_domain_desc* d, int64_t n_dom);
-// Append one meshed domain (HEX08 micro) to the batch topology.  d.modulus:
-// modulus_dev: optional per-voxel moduli on the device (n^3), otherwise d.E;
-// escale multiplies every modulus.
+// Append one meshed domain (HEX08 micro) to the batch topology as the group of
+// 24 macro load cases starting at lc_base (load cases past the macro element's
+// count are zero).  modulus_dev: optional per-voxel moduli on the device (n^3),
+// otherwise d.E; escale multiplies every modulus.
 int batch_add_domain(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d, const double* modulus_dev,
-                     double escale, hipStream_t s);
+                     double escale, int lc_base, hipStream_t s);
 // Append a domain that failed before the solve (empty mesh, ...).
-int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status);
+int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status, int lc_base);
 // Allocate vectors and scalars once all domains are added.
 int batch_finalize(SolverBatch& b, int precision);
 // Build the coarse levels of the multigrid preconditioner (after batch_finalize).
@@ -92,10 +93,11 @@ int batch_build_mg(SolverBatch& b, int max_levels);
 // Full solve: returns per-domain status in b.h_status; fills iteration counts.
 int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& iters,
                 std::vector<int64_t>& outer, std::vector<double>& relres);
-// ff (24x24 per domain, [lc][dof]) after a solve.
+// ff per domain group, [24 lc][60 macro DOF slots] (first 3 x nodes slots used), after a solve.
 int batch_forces(SolverBatch& b, std::vector<double>& ff);
-// displacements / reactions of domain i in mesh numbering, column-major (3N, 24).
-int batch_fields(SolverBatch& b, int i, double* u_out, double* r_out);
+// displacements / reactions of domain group i in mesh numbering, column-major
+// (3N, ncol): the first ncol load cases of the group.
+int batch_fields(SolverBatch& b, int i, int ncol, double* u_out, double* r_out);
 // sensitivity dL/dE per voxel of domain i.
 int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host);
 

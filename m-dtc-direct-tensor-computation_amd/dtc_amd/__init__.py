@@ -104,17 +104,19 @@ def mesh(phi, llimit, nen=8, delta=(1.0, 1.0, 1.0), xa_n=(1, 1, 1)):
 
 
 def boundaries(nodes, min_c, max_c, dim_c, macro_order=1):
-    """Returns (ids (n_b,), vals (24, 3*n_b)) like bnode_vals(lc, 3*(b-1)+d)."""
+    """Returns (ids (n_b,), vals (R, 3*n_b)) like bnode_vals(lc, 3*(b-1)+d), R = 24
+    (HEX08 macro element) or 60 (HEX20)."""
     nodes = np.ascontiguousarray(nodes, np.float64)
     nb = C.c_int64()
     lib = load()
+    R = _lcs(macro_order)
     check("dtc_boundaries", lib.dtc_boundaries(_p(nodes), nodes.shape[0], _p(_f64(*min_c)), _p(_f64(*max_c)),
                                                _p(_f64(*dim_c)), macro_order, C.byref(nb), None, None))
     ids = np.zeros(nb.value, np.int64)
-    vals = np.zeros(24 * 3 * nb.value)
+    vals = np.zeros(R * 3 * nb.value)
     check("dtc_boundaries", lib.dtc_boundaries(_p(nodes), nodes.shape[0], _p(_f64(*min_c)), _p(_f64(*max_c)),
                                                _p(_f64(*dim_c)), macro_order, C.byref(nb), _p(ids), _p(vals)))
-    return ids, vals.reshape(3 * nb.value, 24).T.copy()
+    return ids, vals.reshape(3 * nb.value, R).T.copy()
 
 
 def element_stiffness(nen, E, nu, a):
@@ -190,11 +192,16 @@ def opts(precision=DTC_FP32_REFINE, rel_tol=1e-8, inner_tol=1e-4, max_iter=20000
     return o
 
 
-def _result(r: DomainResult) -> dict:
-    a = lambda x, *s: np.array(x[:], dtype=np.float64).reshape(*s)
+def _lcs(macro_order):
+    return 24 if macro_order == 1 else 60
+
+
+def _result(r: DomainResult, macro_order=1) -> dict:
+    a = lambda x, *s: np.array(x[:int(np.prod(s))], dtype=np.float64).reshape(*s)
+    R = _lcs(macro_order)
     out = dict(status=r.status, n_elems=r.n_elems, n_nodes=r.n_nodes, n_bnodes=r.n_bnodes, iterations=r.iterations,
                outer_iterations=r.outer_iterations, relres=r.relres_max,
-               ff=a(r.ff, 24, 24).T.copy(), K_macro=a(r.K_macro, 24, 24).T.copy(), C=a(r.C, 6, 6).T.copy(),
+               ff=a(r.ff, R, R).T.copy(), K_macro=a(r.K_macro, R, R).T.copy(), C=a(r.C, 6, 6).T.copy(),
                EE=a(r.EE, 6, 6).T.copy(), density=r.density)
     out["rot"] = [dict(alpha=r.rot_alpha[c], n=np.array(r.rot_n[3 * c:3 * c + 3]),
                        aa=np.array(r.rot_aa[9 * c:9 * c + 9]).reshape(3, 3).T.copy(),
@@ -217,7 +224,7 @@ class Batch:
     def run(self):
         res = (DomainResult * len(self.domains))()
         check("dtc_batch_run", load().dtc_batch_run(self._h, C.byref(res)))
-        return [_result(r) for r in res]
+        return [_result(r, d.macro_order) for r, d in zip(res, self.domains)]
 
     def grad(self, i, dL_dC):
         d = self.domains[i]
@@ -257,16 +264,17 @@ def solve_domain(domain: Domain, want_u=False, want_r=False, **kw):
     nn = None
     if want_u or want_r:
         _, nn = mesh_count(domain.phi, domain.llimit, domain.micro_nen)
-    u = np.zeros(3 * nn * 24) if want_u else None
-    r = np.zeros(3 * nn * 24) if want_r else None
+    R = _lcs(domain.macro_order)
+    u = np.zeros(3 * nn * R) if want_u else None
+    r = np.zeros(3 * nn * R) if want_r else None
     check("dtc_solve_domain", load().dtc_solve_domain(C.byref(d), C.byref(o), C.byref(res),
                                                       _p(u) if u is not None else None,
                                                       _p(r) if r is not None else None))
-    out = _result(res)
+    out = _result(res, domain.macro_order)
     if want_u:
-        out["u"] = u.reshape(24, 3 * nn)        # u[lc, 3*(node-1)+d]
+        out["u"] = u.reshape(R, 3 * nn)         # u[lc, 3*(node-1)+d]
     if want_r:
-        out["r"] = r.reshape(24, 3 * nn)
+        out["r"] = r.reshape(R, 3 * nn)
     return out
 
 
@@ -275,7 +283,7 @@ def solve_batch(domains: Sequence[Domain], **kw):
     o = opts(**kw)
     res = (DomainResult * len(domains))()
     check("dtc_solve_batch", load().dtc_solve_batch(C.byref(descs), len(domains), C.byref(o), C.byref(res)))
-    return [_result(r) for r in res]
+    return [_result(r, d.macro_order) for r, d in zip(res, domains)]
 
 
 def rotation_search(EE, criterion, step_deg=1.0, num_zero=1e-9):

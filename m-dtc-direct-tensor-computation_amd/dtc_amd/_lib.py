@@ -51,7 +51,7 @@ class DomainResult(C.Structure):
     _fields_ = [
         ("status", C.c_int), ("_pad", C.c_int), ("n_elems", I64), ("n_nodes", I64), ("n_bnodes", I64),
         ("iterations", I64), ("outer_iterations", I64), ("relres_max", C.c_double),
-        ("ff", C.c_double * 576), ("K_macro", C.c_double * 576), ("C", C.c_double * 36), ("EE", C.c_double * 36),
+        ("ff", C.c_double * 3600), ("K_macro", C.c_double * 3600), ("C", C.c_double * 36), ("EE", C.c_double * 36),
         ("density", C.c_double), ("rot_alpha", C.c_double * 2), ("rot_n", C.c_double * 6),
         ("rot_aa", C.c_double * 18), ("rot_EE", C.c_double * 72), ("rot_jj", I64 * 2), ("rot_crit", C.c_double * 2),
     ]

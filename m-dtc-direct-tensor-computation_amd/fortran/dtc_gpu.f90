@@ -57,7 +57,8 @@ module dtc_gpu
      integer(c_int)       :: status, pad
      integer(c_int64_t)   :: n_elems, n_nodes, n_bnodes, iterations, outer_iterations
      real(c_double)       :: relres_max
-     real(c_double)       :: ff(24,24), K_macro(24,24), C(6,6), EE(6,6)
+     real(c_double)       :: ff(3600), K_macro(3600)   ! column-major R x R, R = 24 or 60
+     real(c_double)       :: C(6,6), EE(6,6)
      real(c_double)       :: density
      real(c_double)       :: rot_alpha(2), rot_n(3,2), rot_aa(3,3,2), rot_EE(6,6,2)
      integer(c_int64_t)   :: rot_jj(2)

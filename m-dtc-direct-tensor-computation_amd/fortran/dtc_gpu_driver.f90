@@ -171,12 +171,15 @@ contains
          "Final coordinate system CR_1", "Optimized Effective stiffness CR_1", "Rotation Angle CR_2", &
          "Rotation Vector CR_2", "Final coordinate system CR_2", "Optimized Effective stiffness CR_2", &
          "Effective density"]
-    integer, parameter :: sz(NL) = [1, 1, 1, 24, 1, 1, 576, 576, 1, 144, 144, 36, 1, 36, 1, 1, 3, 9, 36, 1, 3, 9, 36, 1]
+    integer :: sz(NL), nlc
     integer(c_int64_t) :: lb(NL), pos_i, pos_r
     integer :: ui, ur, uh, l, n
     integer(c_int64_t) :: ilog(24)
     real(c_double), allocatable :: zero(:)
 
+    ! leaf sizes with no_lc = 24 (HEX08 macro element) or 60 (HEX20)
+    nlc = merge(60, 24, macro_order == 2)
+    sz = [1, 1, 1, 24, 1, 1, nlc*nlc, nlc*nlc, 1, 6*nlc, 6*nlc, 36, 1, 36, 1, 1, 3, 9, 36, 1, 3, 9, 36, 1]
     pos_i = 1; pos_r = 1
     do l = 1, NL
        if (l <= 4) then
@@ -187,7 +190,7 @@ contains
     end do
     open(newunit=ui, file=stem//".int8.st", access="stream", form="unformatted", status="replace")
     open(newunit=ur, file=stem//".real8.st", access="stream", form="unformatted", status="replace")
-    allocate(zero(144)); zero = 0._c_double
+    allocate(zero(360)); zero = 0._c_double
     ilog = 0
     do n = 1, nd
        ! element offsets lbound-1+(comm_nn-1)*size, 8 bytes each
@@ -197,11 +200,11 @@ contains
        write(ui, pos=8*(lb(4)-1+(n-1)*sz(4))+1) ilog
        write(ur, pos=8*(lb(5)-1+(n-1)*sz(5))+1) zero(1:1)
        write(ur, pos=8*(lb(6)-1+(n-1)*sz(6))+1) zero(1:1)
-       write(ur, pos=8*(lb(7)-1+(n-1)*sz(7))+1) res(n)%ff
-       write(ur, pos=8*(lb(8)-1+(n-1)*sz(8))+1) res(n)%K_macro
+       write(ur, pos=8*(lb(7)-1+(n-1)*sz(7))+1) res(n)%ff(1:sz(7))
+       write(ur, pos=8*(lb(8)-1+(n-1)*sz(8))+1) res(n)%K_macro(1:sz(8))
        write(ur, pos=8*(lb(9)-1+(n-1)*sz(9))+1) zero(1:1)
-       write(ur, pos=8*(lb(10)-1+(n-1)*sz(10))+1) zero(1:144)
-       write(ur, pos=8*(lb(11)-1+(n-1)*sz(11))+1) zero(1:144)
+       write(ur, pos=8*(lb(10)-1+(n-1)*sz(10))+1) zero(1:sz(10))
+       write(ur, pos=8*(lb(11)-1+(n-1)*sz(11))+1) zero(1:sz(11))
        write(ur, pos=8*(lb(12)-1+(n-1)*sz(12))+1) res(n)%C
        write(ur, pos=8*(lb(13)-1+(n-1)*sz(13))+1) zero(1:1)
        write(ur, pos=8*(lb(14)-1+(n-1)*sz(14))+1) res(n)%EE

@@ -348,13 +348,25 @@ static const struct { int lc, d, node; double m; } UU8[] = {
     {24, 2, 6, 1}, {24, 2, 7, -1},
 };
 
+/* init_displ_hexe20, f-src/mod_write_deck.f90:118-279 (same entry layout; the
+ * table is generated from the parsed reference literals, see the include) */
+#include "uu20_table.inc"
+
 int dtco_uu(int macro_order, double* uu) {
-    if (macro_order != 1) return 5;
     const double eps = 1.e-6;
-    memset(uu, 0, sizeof(double) * 8 * 3 * 24);
-    for (size_t i = 0; i < sizeof(UU8) / sizeof(UU8[0]); ++i)
-        uu[(UU8[i].node - 1) + 8 * ((UU8[i].d - 1) + 3 * (UU8[i].lc - 1))] = UU8[i].m * eps;
-    return 0;
+    if (macro_order == 1) {
+        memset(uu, 0, sizeof(double) * 8 * 3 * 24);
+        for (size_t i = 0; i < sizeof(UU8) / sizeof(UU8[0]); ++i)
+            uu[(UU8[i].node - 1) + 8 * ((UU8[i].d - 1) + 3 * (UU8[i].lc - 1))] = UU8[i].m * eps;
+        return 0;
+    }
+    if (macro_order == 2) {
+        memset(uu, 0, sizeof(double) * 20 * 3 * 60);
+        for (size_t i = 0; i < sizeof(UU20) / sizeof(UU20[0]); ++i)
+            uu[(UU20[i].node - 1) + 20 * ((UU20[i].d - 1) + 3 * (UU20[i].lc - 1))] = UU20[i].m * eps;
+        return 0;
+    }
+    return 5;
 }
 
 /* init_loadcase_el_order_lin, f-src/mod_struct_calcmat.f90:1684-1770: the
@@ -386,10 +398,10 @@ int dtco_boundaries(const double* nodes, int64_t n_nodes, const double min_c[3],
                     const double max_c[3], const double dim_c[3], int macro_order,
                     int64_t* n_b_out, int64_t** ids_out, double** vals_out) {
     *n_b_out = 0; *ids_out = NULL; *vals_out = NULL;
-    if (macro_order != 1) return 5;
-    const int R = 24, nenM = 8;
-    double uu[8 * 3 * 24];
-    dtco_uu(1, uu);
+    if (macro_order != 1 && macro_order != 2) return 5;
+    const int nenM = macro_order == 1 ? 8 : 20, R = 3 * nenM;
+    double uu[20 * 3 * 60];
+    dtco_uu(macro_order, uu);
     int64_t nb = 0;
     for (int64_t i = 0; i < n_nodes; ++i) nb += on_face(nodes + 3 * i, min_c, max_c, dim_c);
     int64_t* ids = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb > 0 ? nb : 1));
@@ -400,8 +412,8 @@ int dtco_boundaries(const double* nodes, int64_t n_nodes, const double min_c[3],
         const double* c = nodes + 3 * i;
         if (!on_face(c, min_c, max_c, dim_c)) continue;
         ids[b] = i + 1;
-        double N[8];
-        dtco_macro_shape(1, c, min_c, max_c, N);
+        double N[20];
+        dtco_macro_shape(macro_order, c, min_c, max_c, N);
         for (int lc = 0; lc < R; ++lc)
             for (int d = 0; d < 3; ++d) {
                 double s = 0.0;                   /* Fortran SUM, left to right */
@@ -613,9 +625,9 @@ int dtco_solve_domain(const double* nodes, int64_t n_nodes, const int64_t* elems
                       const double dim_c[3], int macro_order, double tol, int64_t max_it,
                       int norm_type, int lc_first, int lc_count,
                       double* u_out, double* r_out, double* ff, int64_t* iters, double* relres) {
-    if (macro_order != 1) return 5;
+    if (macro_order != 1 && macro_order != 2) return 5;
     if (n_nodes <= 0 || n_elems <= 0) return 1;
-    const int R = 24, nenM = 8;
+    const int nenM = macro_order == 1 ? 8 : 20, R = 3 * nenM;
     if (lc_first < 0 || lc_count <= 0 || lc_first + lc_count > R) return 5;
     const int64_t m = 3 * n_nodes;
     double* Ke = (double*)malloc(sizeof(double) * 3600);
@@ -694,8 +706,8 @@ int dtco_solve_domain(const double* nodes, int64_t n_nodes, const int64_t* elems
         /* ff((k-1)*nenM + l, lc) = sum over all nodes r_k N_l (f-src/mod_struct_calcmat.f90:254-280) */
         if (ff) {
             for (int64_t nd = 0; nd < n_nodes; ++nd) {
-                double N[8];
-                dtco_macro_shape(1, nodes + 3 * nd, min_c, max_c, N);
+                double N[20];
+                dtco_macro_shape(macro_order, nodes + 3 * nd, min_c, max_c, N);
                 for (int k = 0; k < 3; ++k)
                     for (int l = 0; l < nenM; ++l) ff[(k * nenM + l) + R * lc] += rf[3 * nd + k] * N[l];
             }
@@ -762,16 +774,51 @@ static void fv_weights_lin(const double xdp[3], double W[6 * 24]) {
 #undef LIST
 }
 
+/* the same for the HEX20 macro element, f-src/mod_struct_calcmat.f90:365-401
+ * (20 nodes per direction block; W is 6 x 60) */
+static void fv_weights_quad(const double xdp[3], double W[6 * 60]) {
+    memset(W, 0, sizeof(double) * 6 * 60);
+    const double A23 = xdp[1] * xdp[2], A13 = xdp[0] * xdp[2], A12 = xdp[0] * xdp[1];
+#define SETW(i, list, val) do { const int L_[] = list; for (size_t t_ = 0; t_ < sizeof(L_)/sizeof(int); ++t_) W[(i) + 6 * (L_[t_] - 1)] += (val); } while (0)
+#define LIST(...) {__VA_ARGS__}
+    SETW(0, LIST(2, 3, 6, 7, 10, 14, 18, 19), 1.0 / A23);
+    SETW(1, LIST(23, 24, 27, 28, 31, 35, 39, 40), 1.0 / A13);
+    SETW(2, LIST(45, 46, 47, 48, 53, 54, 55, 56), 1.0 / A12);
+    SETW(3, LIST(3, 4, 7, 8, 11, 15, 19, 20), 0.5 / A13);
+    SETW(3, LIST(22, 23, 26, 27, 30, 34, 38, 39), 0.5 / A23);
+    SETW(4, LIST(5, 6, 7, 8, 13, 14, 15, 16), 0.5 / A12);
+    SETW(4, LIST(42, 43, 46, 47, 50, 54, 58, 59), 0.5 / A23);
+    SETW(5, LIST(25, 26, 27, 28, 33, 34, 35, 36), 0.5 / A12);
+    SETW(5, LIST(43, 44, 47, 48, 51, 55, 59, 60), 0.5 / A13);
+#undef SETW
+#undef LIST
+}
+
 static const double MEPS[6] = {1.e6, 1.e6, 1.e6, -1.e6, -1.e6, 1.e6};   /* :434-441 */
+
+/* HEX20 load-case matrix vv(dof, lc) = uu(l, d, lc) * RVE_STRAIN / 1e-6 with dof =
+ * (d-1)*20 + l.  The reference's own table (init_loadcase_el_order_quad,
+ * f-src/mod_struct_calcmat.f90:1780-2028) leaves row 49 unassigned and holds
+ * unscaled literals, so it is singular; K_M of the HEX20 macro element is formed
+ * with this matrix instead (documented deviation). */
+int dtco_vv_quad_consistent(double rve_strain, double* vv) {
+    double uu[20 * 3 * 60];
+    dtco_uu(2, uu);
+    for (int lc = 0; lc < 60; ++lc)
+        for (int d = 0; d < 3; ++d)
+            for (int l = 0; l < 20; ++l) vv[(d * 20 + l) + 60 * lc] = uu[l + 20 * (d + 3 * lc)] * rve_strain / 1.e-6;
+    return 0;
+}
 
 int dtco_effective(const double* ff, int macro_order, double rve_strain,
                    const double x_D_phy[3], double* K_M, double* C, double* EE) {
-    if (macro_order != 1) return 5;
-    const int R = 24;
+    if (macro_order != 1 && macro_order != 2) return 5;
+    const int R = macro_order == 1 ? 24 : 60;
     int rc = 0;
     if (K_M) {
-        double vv[24 * 24];
-        dtco_vv_lin(rve_strain, vv);
+        double vv[60 * 60];
+        if (macro_order == 1) dtco_vv_lin(rve_strain, vv);
+        else dtco_vv_quad_consistent(rve_strain, vv);
         rc = invert(vv, R);
         if (!rc)
             for (int i = 0; i < R; ++i)
@@ -782,8 +829,9 @@ int dtco_effective(const double* ff, int macro_order, double rve_strain,
                 }
     }
     /* fv(i, lc) for lc = 1..6 */
-    double W[6 * 24], fv[36];
-    fv_weights_lin(x_D_phy, W);
+    double W[6 * 60], fv[36];
+    if (macro_order == 1) fv_weights_lin(x_D_phy, W);
+    else fv_weights_quad(x_D_phy, W);
     /* the reference sums each group left to right before dividing; this is a
      * different FP order of the same expression (tolerance parity) */
     for (int i = 0; i < 6; ++i)

@@ -51,6 +51,8 @@ int dtco_macro_shape(int macro_order, const double x[3], const double min_c[3],
 int dtco_uu(int macro_order, double* uu);
 /* load-case matrix vv(dof,lc) for HEX08 macro: f-src/mod_struct_calcmat.f90:1684-1770 */
 int dtco_vv_lin(double rve_strain, double* vv);
+/* HEX20 macro: vv(dof, lc) = uu * RVE_STRAIN / 1e-6, 60 x 60 (see dtco_effective) */
+int dtco_vv_quad_consistent(double rve_strain, double* vv);
 
 /* a7: element stiffness by Gauss quadrature (2x2x2 HEX08, 3x3x3 HEX20) in
  * gen_quadmesh node order; equals f-src/mod_linfe.f90:161-766 / :890-4509. */
@@ -73,8 +75,10 @@ int dtco_solve_domain(const double* nodes, int64_t n_nodes, const int64_t* elems
                       int norm_type, int lc_first, int lc_count,
                       double* u, double* r, double* ff, int64_t* iters, double* relres);
 
-/* a10/a11: K_M = ff vv^-1 (HEX08 macro only), fv, C = fv meps, EE = (C+C^T)/2
- * (f-src/mod_struct_calcmat.f90:216-303, :340-473).  K_M may be NULL. */
+/* a10/a11: K_M = ff vv^-1, fv, C = fv meps, EE = (C+C^T)/2
+ * (f-src/mod_struct_calcmat.f90:216-303, :340-473), R = 24 (macro_order 1) or
+ * 60 (macro_order 2).  K_M may be NULL.  For the HEX20 macro element vv is
+ * dtco_vv_quad_consistent (the reference's own table is singular). */
 int dtco_effective(const double* ff, int macro_order, double rve_strain,
                    const double x_D_phy[3], double* K_M, double* C, double* EE);
 /* density: f-src/mod_struct_calcmat.f90:1642-1646 */

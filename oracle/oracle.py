@@ -105,7 +105,7 @@ def boundaries(nodes, min_c, max_c, dim_c, macro_order=1):
     _chk("boundaries", lib().dtco_boundaries(_p(nodes), C.c_int64(nodes.shape[0]), _p(_f64(*min_c)),
                                              _p(_f64(*max_c)), _p(_f64(*dim_c)), C.c_int(macro_order),
                                              C.byref(nb), C.byref(pi), C.byref(pv)))
-    R = 24
+    R = 24 if macro_order == 1 else 60
     ids = np.ctypeslib.as_array(C.cast(pi, C.POINTER(C.c_int64)), shape=(nb.value,)).copy()
     vals = np.ctypeslib.as_array(C.cast(pv, C.POINTER(C.c_double)), shape=(3 * nb.value, R)).copy()
     lib().dtco_free(pi); lib().dtco_free(pv)
@@ -120,15 +120,22 @@ def macro_shape(x, min_c, max_c, macro_order=1):
 
 
 def uu(macro_order=1):
-    out = np.zeros(8 * 3 * 24)
+    nen = 8 if macro_order == 1 else 20
+    out = np.zeros(nen * 3 * 3 * nen)
     _chk("uu", lib().dtco_uu(C.c_int(macro_order), _p(out)))
-    return out.reshape(24, 3, 8).transpose(2, 1, 0).copy()   # uu[l, d, lc]
+    return out.reshape(3 * nen, 3, nen).transpose(2, 1, 0).copy()   # uu[l, d, lc]
 
 
 def vv_lin(rve_strain):
     out = np.zeros(24 * 24)
     _chk("vv", lib().dtco_vv_lin(C.c_double(rve_strain), _p(out)))
     return out.reshape(24, 24).T.copy()                      # vv[dof, lc]
+
+
+def vv_quad_consistent(rve_strain):
+    out = np.zeros(60 * 60)
+    _chk("vv", lib().dtco_vv_quad_consistent(C.c_double(rve_strain), _p(out)))
+    return out.reshape(60, 60).T.copy()                      # vv[dof, lc]
 
 
 def ke(nen, E, nu, a):
@@ -139,16 +146,19 @@ def ke(nen, E, nu, a):
 
 
 def solve_domain(nodes, elems, E, nu, a, min_c, max_c, dim_c, tol=1e-10, max_it=100000,
-                 norm_type=0, emod=None, lc_first=0, lc_count=24, want_u=False, want_r=False,
+                 norm_type=0, emod=None, lc_first=0, lc_count=None, want_u=False, want_r=False,
                  macro_order=1):
     nodes = np.ascontiguousarray(nodes, np.float64)
     elems = np.ascontiguousarray(elems, np.int64)
     nn, nen = nodes.shape[0], elems.shape[1]
     m = 3 * nn
-    u = np.zeros((24, m)) if want_u else None
-    r = np.zeros((24, m)) if want_r else None
-    ff = np.zeros((24, 24))
-    iters = np.zeros(24, np.int64); relres = np.zeros(24)
+    R = 24 if macro_order == 1 else 60
+    if lc_count is None:
+        lc_count = R - lc_first
+    u = np.zeros((R, m)) if want_u else None
+    r = np.zeros((R, m)) if want_r else None
+    ff = np.zeros((R, R))
+    iters = np.zeros(R, np.int64); relres = np.zeros(R)
     em = None if emod is None else np.ascontiguousarray(emod, np.float64)
     _chk("solve_domain", lib().dtco_solve_domain(
         _p(nodes), C.c_int64(nn), _p(elems), C.c_int64(elems.shape[0]), C.c_int(nen),
@@ -162,10 +172,11 @@ def solve_domain(nodes, elems, E, nu, a, min_c, max_c, dim_c, tol=1e-10, max_it=
 
 def effective(ff, x_D_phy, rve_strain=1e-6, macro_order=1):
     ffc = np.ascontiguousarray(np.asarray(ff, np.float64).T)      # column-major
-    KM = np.zeros(24 * 24); Cm = np.zeros(36); EE = np.zeros(36)
+    R = ffc.shape[0]
+    KM = np.zeros(R * R); Cm = np.zeros(36); EE = np.zeros(36)
     _chk("effective", lib().dtco_effective(_p(ffc), C.c_int(macro_order), C.c_double(rve_strain),
                                            _p(_f64(*x_D_phy)), _p(KM), _p(Cm), _p(EE)))
-    return KM.reshape(24, 24).T.copy(), Cm.reshape(6, 6).T.copy(), EE.reshape(6, 6).T.copy()
+    return KM.reshape(R, R).T.copy(), Cm.reshape(6, 6).T.copy(), EE.reshape(6, 6).T.copy()
 
 
 def density(n_elems, x_D_phy, delta):

@@ -36,16 +36,30 @@ _OPS = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul,
         ast.Div: operator.truediv, ast.Pow: operator.pow, ast.USub: operator.neg, ast.UAdd: operator.pos}
 
 
+_REAL = re.compile(r"(?<![\w.])(\d+\.\d*|\.\d+)([eEdD][+-]?\d+)?(_rk|_8)?(?![\w.])")
+
+
 def _eval(expr: str, names: dict) -> float:
-    """Arithmetic-only evaluation of a Fortran real expression."""
-    e = re.sub(r"_rk\b|_8\b", "", expr)
-    e = re.sub(r"(\d)\.(?![\d])", r"\1.0", e)       # 0._rk -> 0.0
-    e = re.sub(r"(?<![\w.])\.(\d)", r"0.\1", e)      # .5 -> 0.5
+    """Arithmetic-only evaluation of a Fortran real expression.  A real literal
+    with a kind suffix (_rk, _8) is double precision; one without is of the
+    default real kind, i.e. single precision (the reference is built without a
+    default-real promotion flag), so `.9*eps` is float32(0.9) * eps."""
+    def lit(m):
+        txt = m[1] + ((m[2] or "").replace("d", "e").replace("D", "e"))
+        if "." in txt and txt.startswith("."):
+            txt = "0" + txt
+        if txt.endswith("."):
+            txt += "0"
+        txt = txt.replace(".e", ".0e")
+        return txt if m[3] else f"_f4({txt})"
+    e = _REAL.sub(lit, expr)
     tree = ast.parse(e.strip(), mode="eval")
 
     def ev(n):
         if isinstance(n, ast.Expression):
             return ev(n.body)
+        if isinstance(n, ast.Call) and isinstance(n.func, ast.Name) and n.func.id == "_f4" and len(n.args) == 1:
+            return float(np.float32(ev(n.args[0])))
         if isinstance(n, ast.Constant) and isinstance(n.value, (int, float)):
             return float(n.value)
         if isinstance(n, ast.Name):

@@ -32,7 +32,7 @@ def test_struct_layouts_match_the_header():
     # sizes implied by the header (all members naturally aligned, no implicit padding)
     assert ctypes.sizeof(L.DomainDesc) == 8 + 4 * 4 + 3 * 8 + 8 + 3 * 8 + 3 * 8 + 8 + 3 * 8 + 8
     assert ctypes.sizeof(L.SolveOpts) == 4 + 4 + 8 + 8 + 8 + 8 + 8 + 4 + 4
-    assert ctypes.sizeof(L.DomainResult) == 8 + 6 * 8 + 576 * 16 + 72 * 8 + 8 + 2 * 8 + 6 * 8 + 18 * 8 + 72 * 8 + 2 * 8 + 2 * 8
+    assert ctypes.sizeof(L.DomainResult) == 8 + 6 * 8 + 3600 * 16 + 72 * 8 + 8 + 2 * 8 + 6 * 8 + 18 * 8 + 72 * 8 + 2 * 8 + 2 * 8
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

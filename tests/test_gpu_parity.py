@@ -293,3 +293,67 @@ def test_multigrid_preconditioned_solve_matches_jacobi_and_oracle():
     d = doms[2]
     o = _oracle_domain(d, tol=1e-11)
     assert _rel(rm[2]["C"], o["C"]) < 1e-5
+
+
+# ------------------------------------------------------------------ HEX20 macro element (60 load cases)
+@pytest.mark.parametrize("seed", [3, 4])
+def test_hex20_macro_boundaries_bit_exact(seed):
+    dtc, O = _dtc(), _O()
+    phi = random_mask((8, 9, 7), 0.5, seed)
+    delta, xa = 0.01, (3, 2, 4)
+    nodes, _ = O.mesh(phi, 1, 8, (delta,) * 3, xa)
+    mn, mx, dim = domain_box((7, 9, 8), delta, xa)
+    io, vo = O.boundaries(nodes, mn, mx, dim, 2)
+    ig, vg = dtc.boundaries(nodes, mn, mx, dim, 2)
+    assert vg.shape == (60, 3 * len(ig))
+    assert np.array_equal(ig, io)
+    assert np.array_equal(vg.view(np.int64), vo.view(np.int64))
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-5)])
+def test_hex20_macro_solid_cube_closed_form(prec, tol):
+    dtc = _dtc()
+    d = dtc.Domain(np.ones((8, 8, 8), np.int8), delta=(0.02,) * 3, xa_n=(3, 3, 3), macro_order=2)
+    r = dtc.solve_domain(d, precision=prec, rel_tol=1e-8, rotation=False)
+    assert r["status"] == 0 and r["relres"] <= 1e-8
+    assert r["ff"].shape == (60, 60)
+    assert _rel(r["C"], iso_voigt(5600.0, 0.3)) < tol
+
+
+@pytest.mark.parametrize("prec,tol,pc", [(0, 1e-6, 0), (1, 1e-5, 0), (1, 1e-5, 1)])
+def test_hex20_macro_porous_domain_matches_oracle(prec, tol, pc):
+    dtc, O = _dtc(), _O()
+    from dtc_amd import synth
+    phi = synth.trabecular(12, seed=71, sigma=1.5, bvtv=0.4)
+    d = dtc.Domain(phi, delta=(0.01,) * 3, xa_n=(2, 2, 2), macro_order=2)
+    r = dtc.solve_domain(d, precision=prec, rel_tol=1e-8, rotation=False, precond=pc, want_u=True, want_r=True)
+    nodes, elems = O.mesh(phi, 1, 8, d.delta, d.xa_n)
+    mn, mx, dim = d.box()
+    o = O.solve_domain(nodes, elems, d.E, d.nu, d.a(), mn, mx, dim, tol=1e-12, want_u=True, want_r=True,
+                       macro_order=2)
+    KM, Cm, EE = O.effective(o["ff"], dim, d.rve_strain, macro_order=2)
+    assert r["status"] == 0 and r["relres"] <= 1e-8
+    assert _rel(r["ff"], o["ff"]) < tol
+    assert _rel(r["K_macro"], KM) < 10 * tol
+    assert _rel(r["C"], Cm) < tol and _rel(r["EE"], EE) < tol
+    assert r["u"].shape == o["u"].shape == (60, 3 * nodes.shape[0])
+    assert _rel(r["u"], o["u"]) < 10 * tol
+    assert np.abs(r["r"] - o["r"]).max() < 10 * tol * np.abs(o["r"]).max()
+
+
+def test_mixed_macro_orders_in_one_batch():
+    dtc = _dtc()
+    from dtc_amd import synth
+    phi = synth.trabecular(11, seed=72, sigma=1.5, bvtv=0.45)
+    d1 = dtc.Domain(phi, delta=(0.01,) * 3)
+    d2 = dtc.Domain(phi, delta=(0.01,) * 3, macro_order=2)
+    rb = dtc.solve_batch([d1, d2, d1], precision=1, rel_tol=1e-8, rotation=True, precond=1)
+    assert [x["status"] for x in rb] == [0, 0, 0]
+    assert rb[0]["ff"].shape == (24, 24) and rb[1]["ff"].shape == (60, 60)
+    # load cases 1-6 are the same affine fields: identical reactions, C within the fit's face weights
+    s1 = dtc.solve_domain(d1, precision=1, rel_tol=1e-8, rotation=False)
+    assert _rel(rb[0]["C"], s1["C"]) < 1e-6 and _rel(rb[2]["C"], s1["C"]) < 1e-6
+    with pytest.raises(Exception):
+        with dtc.Batch([d2], precision=1) as b:
+            b.run()
+            b.grad(0, np.eye(6))
