@@ -147,8 +147,14 @@ def main():
     # ---- roofline of the dominant kernel (EbE matvec of the FP32 PCG)
     mv_s = st["mv_avg_ms"] * 1e-3
     achieved = st["mv_bytes"] / mv_s / 1e9 if mv_s > 0 else 0.0
+    traffic = None
+    tj = os.path.join(ROOT, "profiles", "k_matvec_traffic.json")
+    if os.path.exists(tj):
+        t = json.load(open(tj))
+        if t.get("batch") == len(doms) and t.get("domain_edge") == args.n:
+            traffic = t["traffic_bytes_per_dispatch"]
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_matvec<float, IN_V, OUT_MASKED_DOT>", "avg_launch_ms": round(st["mv_avg_ms"], 5),
             "launches_timed": st["mv_launches"], "algorithmic_bytes": st["mv_bytes"],
             "dense_equiv_tflops": round(st["mv_flops"] / mv_s / 1e12, 2) if mv_s > 0 else 0.0}
