@@ -237,7 +237,8 @@ int validate_desc(const dtc_domain_desc& d) {
         if (d.n[k] <= 0 || d.n[k] > 4096) return set_error(DTC_ERR_BAD_ARGUMENT, "domain extent n[%d] = %lld", k, (long long)d.n[k]);
         if (!(d.delta[k] > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "grid spacing delta[%d] = %g", k, d.delta[k]);
     }
-    if (d.micro_nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "the solve implements HEX08 micro elements (got %d nodes)", d.micro_nen);
+    if (d.micro_nen != 8 && d.micro_nen != 20)
+        return set_error(DTC_ERR_BAD_ARGUMENT, "micro element with %d nodes (expected 8 or 20)", d.micro_nen);
     if (d.macro_order != 1 && d.macro_order != 2) return set_error(DTC_ERR_BAD_ARGUMENT, "macro element order %d", d.macro_order);
     if (!(d.nu > -1.0 && d.nu < 0.5) || !(d.E > 0.0) || !(d.size_domain > 0.0))
         return set_error(DTC_ERR_BAD_ARGUMENT, "material E=%g nu=%g size_domain=%g", d.E, d.nu, d.size_domain);
@@ -419,6 +420,8 @@ int dtc_batch_create(const dtc_domain_desc* d, int64_t n_dom, const dtc_solve_op
     for (int64_t i = 0; i < n_dom; ++i) DTC_TRY(validate_desc(d[i]));
     for (int64_t i = 1; i < n_dom; ++i)
         if (d[i].nu != d[0].nu) return set_error(DTC_ERR_UNSUPPORTED, "one Poisson ratio per batch");
+    for (int64_t i = 1; i < n_dom; ++i)
+        if (d[i].micro_nen != d[0].micro_nen) return set_error(DTC_ERR_UNSUPPORTED, "one micro element type per batch");
     if (o->precision != DTC_FP64 && o->precision != DTC_FP32_REFINE) return set_error(DTC_ERR_BAD_ARGUMENT, "precision %d", o->precision);
     if (!(o->rel_tol > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "rel_tol %g", o->rel_tol);
     DTC_TRY(check_device());
@@ -449,6 +452,11 @@ int dtc_batch_create(const dtc_domain_desc* d, int64_t n_dom, const dtc_solve_op
     b->sb.stream = b->st.s;
     b->sb.nu = d[0].nu;
     b->sb.a = a0;
+    b->sb.micro20 = d[0].micro_nen == 20;
+    if (b->sb.micro20) {
+        b->sb.khat20.assign(3600, 0.0);
+        DTC_TRY(element_stiffness20(1.0, d[0].nu, a0, b->sb.khat20.data()));
+    }
     *out = b.release();
     return DTC_OK;
 }
@@ -479,12 +487,16 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
             continue;
         }
         DTC_TRY(rc);
-        for (int g = 0; g < b->ng[i]; ++g)
-            DTC_TRY(batch_add_domain(b->sb, b->mesh, d, b->mod[i]->p ? b->mod[i]->as<double>() : nullptr, b->escale[i],
-                                     R * g, s));
+        const double* mod = b->mod[i]->p ? b->mod[i]->as<double>() : nullptr;
+        for (int g = 0; g < b->ng[i]; ++g) {
+            if (b->sb.micro20) DTC_TRY(batch_add_domain20(b->sb, b->mesh, d, mod, b->escale[i], R * g, s));
+            else DTC_TRY(batch_add_domain(b->sb, b->mesh, d, mod, b->escale[i], R * g, s));
+        }
     }
     DTC_TRY(batch_finalize(b->sb, b->o.precision));
-    b->sb.precond = (b->o.precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG) ? DTC_PC_MG : DTC_PC_JACOBI;
+    // the multigrid hierarchy coarsens the voxel lattice: HEX20 micro meshes use Jacobi
+    b->sb.precond = (b->o.precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG && !b->sb.micro20) ? DTC_PC_MG
+                                                                                                      : DTC_PC_JACOBI;
     b->sb.mg_degree = b->o.mg_degree > 0 ? b->o.mg_degree : 2;
     if (b->sb.precond == DTC_PC_MG) DTC_TRY(batch_build_mg(b->sb, 8));
     DTC_TRY(batch_solve(b->sb, b->o, b->iters, b->outer, b->relres));
@@ -557,6 +569,7 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
     if (i < 0 || i >= (int64_t)b->d.size()) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %lld", (long long)i);
     const dtc_domain_desc& d = b->d[i];
     if (d.macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements the HEX08 macro element");
+    if (d.micro_nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements HEX08 micro elements");
     const int gi = b->g0[i];
     const size_t nv = (size_t)d.n[0] * d.n[1] * d.n[2];
     if (b->sb.h_status[gi] != DTC_OK && b->sb.h_status[gi] != DTC_ERR_NO_CONVERGENCE) {

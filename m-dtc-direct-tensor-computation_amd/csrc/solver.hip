@@ -384,7 +384,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                     vzero(dn);
                     vzero(xn);
                     if (!bc) {
-                        const T di = reinterpret_cast<const T*>(A.dinv)[g_off[j] / (3 * NCH)];
+                        const T di = reinterpret_cast<const T*>(A.dinv)[g_off[j] / NCH];
                         const V dv = *dp, xv = s_x[g_xi[j]];
                         T* dnp = reinterpret_cast<T*>(&dn);
                         T* xnp = reinterpret_cast<T*>(&xn);
@@ -453,9 +453,8 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_init(const BrickInfo* __restrict_
         const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
         for (int64_t i = tid; i < cnt; i += NT_UPD) {
             int64_t idx = base + i;
-            int64_t node = idx / (3 * R);
             T rv = r[idx];
-            T z = act ? (zin ? zin[idx] : rv * dinv[node]) : T(0);
+            T z = act ? (zin ? zin[idx] : rv * dinv[idx / R]) : T(0);
             if (act && sp) p[idx] = z;
             if (zero_x) x[idx] = T(0);
             a1 += (double)rv * (double)z;
@@ -487,14 +486,13 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_update(const BrickInfo* __restric
     const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
     for (int64_t i = tid; i < cnt; i += NT_UPD) {
         int64_t idx = base + i;
-        int64_t node = idx / (3 * R);
         T rv = r[idx];
         if (a != T(0)) {      // frozen load cases keep x and r
             rv -= a * q[idx];
             x[idx] += a * p[idx];
             r[idx] = rv;
         }
-        if (prz) a1 += (double)rv * (double)(rv * dinv[node]);
+        if (prz) a1 += (double)rv * (double)(rv * dinv[idx / R]);
         a2 += (double)rv * (double)rv;
     }
     s1[tid] = a1; s2[tid] = a2;
@@ -519,8 +517,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cg_dir(const BrickInfo* __restrict__
     const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
     for (int64_t i = tid; i < cnt; i += NT_UPD) {
         int64_t idx = base + i;
-        int64_t node = idx / (3 * R);
-        p[idx] = (zin ? zin[idx] : r[idx] * dinv[node]) + b * p[idx];
+        p[idx] = (zin ? zin[idx] : r[idx] * dinv[idx / R]) + b * p[idx];
     }
 }
 
@@ -794,8 +791,10 @@ __global__ void k_dinv(const BrickInfo* __restrict__ bricks, const DomInfo* __re
     double dg = kdiag * s;
     double inv = dg != 0.0 ? 1.0 / dg : 1.0;   // PCJACOBI uses 1 for a zero diagonal
     if (is_bc(D, gx, gy, gz)) inv = 0.0;        // constrained DOFs stay zero
-    d64[q] = inv;
-    d32[q] = (float)inv;
+    for (int d = 0; d < 3; ++d) {              // one value per DOF (isotropic for the HEX08 voxel)
+        d64[3 * q + d] = inv;
+        d32[3 * q + d] = (float)inv;
+    }
 }
 
 // ---------------------------------------------------------------- matvec tables
@@ -1229,7 +1228,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cheb(const BrickInfo* __restrict__ b
     for (int64_t i = tid; i < cnt; i += NT_UPD) {
         const int64_t idx = base + i;
         const float bv = b[idx];
-        const float di = dinv[idx / (3 * R)];
+        const float di = dinv[idx / R];
         float dv, xv;
         if (zero) {
             dv = c2 * di * bv;
@@ -1355,6 +1354,7 @@ int batch_begin(SolverBatch& b, const dtc_domain_desc* d, int64_t n_dom) {
     DTC_HIP(hipGetLastError());
     b.n_dom = 0; b.n_nodes = 0; b.n_lat = 0; b.n_vox = 0;
     b.h_dom.clear(); b.h_brick.clear(); b.h_elems.clear(); b.h_nodes.clear(); b.h_bnodes.clear(); b.h_status.clear();
+    b.d20.clear();
     return DTC_OK;
 }
 
@@ -1375,6 +1375,7 @@ int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status, int lc
     b.h_nodes.push_back(0);
     b.h_bnodes.push_back(0);
     b.h_status.push_back(status);
+    if (b.micro20) b.d20.emplace_back();
     return DTC_OK;
 }
 
@@ -1539,8 +1540,8 @@ int batch_finalize(SolverBatch& b, int precision) {
         DTC_HIP(hipMemcpyAsync(b.brick.p, b.h_brick.data(), b.h_brick.size() * sizeof(BrickInfo), hipMemcpyHostToDevice, s));
     const int64_t nn = b.n_nodes > 0 ? b.n_nodes : 1;
     const size_t v64 = (size_t)nn * 3 * R * 8, v32 = (size_t)nn * 3 * R * 4;
-    DTC_TRY(b.dinv64.ensure((size_t)nn * 8));
-    DTC_TRY(b.dinv32.ensure((size_t)nn * 4));
+    DTC_TRY(b.dinv64.ensure((size_t)nn * 3 * 8));
+    DTC_TRY(b.dinv32.ensure((size_t)nn * 3 * 4));
     DTC_TRY(b.x64.ensure(v64));
     DTC_TRY(b.r64.ensure(v64));
     if (precision == DTC_FP64) {
@@ -1574,7 +1575,9 @@ int batch_finalize(SolverBatch& b, int precision) {
     // element-by-element bound lambda_max(D^-1 K) <= lambda_max(diag(Khat)^-1/2 Khat diag(Khat)^-1/2);
     // the diagonal of the cube element is constant
     b.cheb_lmax = sym_lmax(Khat, 24) / b.kdiag;
-    if (b.n_bricks > 0) {
+    if (b.micro20) {
+        DTC_TRY(finalize20(b));
+    } else if (b.n_bricks > 0) {
         k_dinv<<<b.n_bricks, BX * BY * BZ, 0, s>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
                                                    b.emod.as<double>(), b.kdiag, b.dinv64.as<double>(), b.dinv32.as<float>());
         DTC_HIP(hipGetLastError());
@@ -1587,6 +1590,7 @@ int batch_finalize(SolverBatch& b, int precision) {
 }
 
 int batch_build_mg(SolverBatch& b, int max_levels) {
+    if (b.micro20) return set_error(DTC_ERR_UNSUPPORTED, "the multigrid preconditioner implements HEX08 micro elements");
     hipStream_t s = b.stream;
     int maxn = 0;
     for (const DomInfo& D : b.h_dom) maxn = std::max(maxn, std::max(D.nx, std::max(D.ny, D.nz)));
@@ -1676,8 +1680,8 @@ int batch_build_mg(SolverBatch& b, int max_levels) {
         }
         DTC_HIP(hipGetLastError());
         const size_t nn = (size_t)(C.n_nodes > 0 ? C.n_nodes : 1);
-        DTC_TRY(C.dinv64.ensure(nn * 8));
-        DTC_TRY(C.dinv32.ensure(nn * 4));
+        DTC_TRY(C.dinv64.ensure(nn * 3 * 8));
+        DTC_TRY(C.dinv32.ensure(nn * 3 * 4));
         for (DevMem* m : {&C.b, &C.x, &C.t, &C.d}) DTC_TRY(m->ensure(nn * 3 * R * 4));
         if (C.n_bricks > 0) {
             k_dinv<<<C.n_bricks, NO, 0, s>>>(C.brick.as<BrickInfo>(), C.dom.as<DomInfo>(), C.lat2node.as<int>(),
@@ -1735,8 +1739,12 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
         DTC_TRY(event_pair(b, &e0, &e1));
         DTC_HIP(hipEventRecord(e0, b.stream));
     }
-    k_matvec<T, IN, OUT><<<b.n_bricks, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
-    DTC_HIP(hipGetLastError());
+    if (b.micro20) {
+        DTC_TRY(mv20(b, sizeof(T) == 8, IN, OUT, xin, yout, part, done));
+    } else {
+        k_matvec<T, IN, OUT><<<b.n_bricks, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
+        DTC_HIP(hipGetLastError());
+    }
     if (t) DTC_HIP(hipEventRecord(e1, b.stream));
     return DTC_OK;
 }
@@ -1779,6 +1787,7 @@ struct MgParams {
     double ratio = 10.0;       // smoothing passes of the fine and intermediate levels
     int coarse_degree = 4;     // coarsest level
     double coarse_ratio = 50.0;
+    double lscale = 1.0;       // multiplies the element-by-element bound of lambda_max
 };
 
 MgParams mg_params() {
@@ -1786,6 +1795,7 @@ MgParams mg_params() {
     if (const char* e = std::getenv("DTC_MG_RATIO")) p.ratio = std::atof(e);
     if (const char* e = std::getenv("DTC_MG_CDEG")) p.coarse_degree = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("DTC_MG_CRATIO")) p.coarse_ratio = std::atof(e);
+    if (const char* e = std::getenv("DTC_MG_LSCALE")) p.lscale = std::atof(e);
     return p;
 }
 
@@ -1828,7 +1838,7 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
     auto smooth = [&](int l, int m, bool zero, double ratio, double* dot) -> int {
         const LvView v = view(l);
         if (v.T.n_bricks == 0) return DTC_OK;
-        const double hi = b.cheb_lmax, lo = hi / ratio;
+        const double hi = b.cheb_lmax * P.lscale, lo = hi / ratio;
         const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
         double rho = 1.0 / sigma;
         for (int k = 0; k < m; ++k) {
@@ -2016,7 +2026,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
         // (or meets the final target, or after REPLACE iterations), r32 is
         // replaced by the true FP64 residual and the search direction p32 is
         // kept.
-        const bool mg = b.precond == DTC_PC_MG;
+        const bool mg = b.precond == DTC_PC_MG && !b.micro20;
         const int64_t REPLACE = mg ? 100 : 1000;
         float* x32 = b.x32.as<float>();
         float* r32 = b.r32.as<float>();
@@ -2076,10 +2086,16 @@ int batch_forces(SolverBatch& b, std::vector<double>& ff) {
     const WK<double> wk64 = cast_wk<double>(b.wk64);
     // reactions on the unmodified operator: rf = K (x + x_bc), stored in r64
     DTC_TRY((launch_mv<double, IN_U, OUT_PLAIN>(b, wk64, b.x64.as<double>(), b.r64.as<double>(), nullptr, nullptr)));
-    k_project<8><<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
-                                                      b.lat2node.as<int>(), b.r64.as<double>(), b.part_ff.as<double>());
-    k_project<20><<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
-                                                       b.lat2node.as<int>(), b.r64.as<double>(), b.part_ff.as<double>());
+    if (b.micro20) {
+        DTC_TRY(project20(b));
+    } else {
+        k_project<8><<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
+                                                          b.lat2node.as<int>(), b.r64.as<double>(),
+                                                          b.part_ff.as<double>());
+        k_project<20><<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
+                                                           b.lat2node.as<int>(), b.r64.as<double>(),
+                                                           b.part_ff.as<double>());
+    }
     k_red_ff<<<D, 256, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_ff.as<double>(), b.ff.as<double>());
     DTC_HIP(hipGetLastError());
     DTC_HIP(hipMemcpyAsync(ff.data(), b.ff.p, ff.size() * 8, hipMemcpyDeviceToHost, b.stream));
@@ -2098,10 +2114,15 @@ int batch_fields(SolverBatch& b, int i, int ncol, double* u_out, double* r_out) 
     const size_t bytes = (size_t)nn * 3 * ncol * 8;
     if (u_out) DTC_TRY(du.alloc(bytes));
     if (r_out) DTC_TRY(dr.alloc(bytes));
-    k_fields<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
-                                                  b.sol2ref.as<int>(), b.x64.as<double>(), b.r64.as<double>(), i, nn, ncol,
-                                                  u_out ? du.as<double>() : nullptr, r_out ? dr.as<double>() : nullptr);
-    DTC_HIP(hipGetLastError());
+    if (b.micro20) {
+        DTC_TRY(fields20(b, i, ncol, u_out ? du.as<double>() : nullptr, r_out ? dr.as<double>() : nullptr));
+    } else {
+        k_fields<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
+                                                      b.lat2node.as<int>(), b.sol2ref.as<int>(), b.x64.as<double>(),
+                                                      b.r64.as<double>(), i, nn, ncol, u_out ? du.as<double>() : nullptr,
+                                                      r_out ? dr.as<double>() : nullptr);
+        DTC_HIP(hipGetLastError());
+    }
     if (u_out) DTC_HIP(hipMemcpyAsync(u_out, du.p, bytes, hipMemcpyDeviceToHost, b.stream));
     if (r_out) DTC_HIP(hipMemcpyAsync(r_out, dr.p, bytes, hipMemcpyDeviceToHost, b.stream));
     DTC_HIP(hipStreamSynchronize(b.stream));
@@ -2110,6 +2131,7 @@ int batch_fields(SolverBatch& b, int i, int ncol, double* u_out, double* r_out) 
 
 int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host) {
     if (i < 0 || i >= b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %d", i);
+    if (b.micro20) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements HEX08 micro elements");
     const DomInfo& D = b.h_dom[i];
     const int64_t nvox = (int64_t)D.nx * D.ny * D.nz;
     const int64_t nn = b.n_nodes > 0 ? b.n_nodes : 1;

@@ -15,6 +15,27 @@ struct Level {
     DevMem b, x, t, d;             // right-hand side, iterate, matvec output, Chebyshev direction
 };
 
+// Mesh of one HEX20 micro-element domain (owning) and its device view.
+struct Dom20 {
+    DevMem coords;                 // double [node][3], mesh order
+    DevMem elems;                  // int [elem][20]: 0-based local node ids
+    DevMem emod;                   // double per element
+    DevMem adj_off, adj;           // node -> sorted (element << 5 | local node)
+    DevMem bcf;                    // uint8 per node: Dirichlet node
+    DevMem scan_tmp;
+    int64_t node_begin = 0, n_nodes = 0;
+};
+struct Dom20Dev {
+    const double* coords;
+    const int* elems;
+    const double* emod;
+    const int* adj_off;
+    const int* adj;
+    const unsigned char* bcf;
+    int64_t node_begin;
+    int n_nodes;
+};
+
 // One batch of domains resident on the device: topology, vectors, scalars.
 struct SolverBatch {
     int n_dom = 0;
@@ -35,7 +56,7 @@ struct SolverBatch {
     DevMem einfo;                  // int per tile element: halo base | (owned base + 41) << 9 | owned corners << 24
     DevMem eE64, eE32;             // element modulus per tile element
     int64_t n_tile_elems = 0;
-    DevMem dinv64, dinv32;         // Jacobi: 1 / (kdiag * sum of adjacent moduli)
+    DevMem dinv64, dinv32;         // Jacobi: 1 / diag(K) per DOF [node][3] (0 on Dirichlet DOFs)
     // vectors, layout [node][3][24]
     DevMem x64, r64, p64, q64;
     DevMem x32, r32, p32, q32;
@@ -57,6 +78,12 @@ struct SolverBatch {
     int precond = 0;               // DTC_PC_JACOBI / DTC_PC_MG
     int mg_degree = 2;             // Chebyshev degree of the fine and intermediate levels
     double cheb_lmax = 0.0;        // element-by-element bound of lambda_max(D^-1 K)
+    // HEX20 micro elements: node-gather operator over pseudo-bricks of mesh nodes
+    bool micro20 = false;
+    std::vector<Dom20> d20;        // per domain group (empty entries for failed groups)
+    DevMem d20dev;                 // Dom20Dev[]
+    std::vector<double> khat20;    // E = 1 element stiffness, 60 x 60
+    DevMem k20_64, k20_32;
     WalshK wk64{};
     double kdiag = 0.0;            // diagonal entry of the E = 1 element stiffness
     double nu = 0.3, a = 1.0;
@@ -98,6 +125,15 @@ int batch_forces(SolverBatch& b, std::vector<double>& ff);
 // displacements / reactions of domain group i in mesh numbering, column-major
 // (3N, ncol): the first ncol load cases of the group.
 int batch_fields(SolverBatch& b, int i, int ncol, double* u_out, double* r_out);
+// HEX20 micro elements (solver20.hip): append a domain group, finish the tables
+// after batch_finalize's allocations, the operator in the HEX08 kernel's modes,
+// the force projection and the fields of one group.
+int batch_add_domain20(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d, const double* modulus_dev,
+                       double escale, int lc_base, hipStream_t s);
+int finalize20(SolverBatch& b);
+int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout, double* part, const int* done);
+int project20(SolverBatch& b);
+int fields20(SolverBatch& b, int i, int ncol, double* u_dev, double* r_dev);
 // sensitivity dL/dE per voxel of domain i.
 int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host);
 

@@ -1,0 +1,412 @@
+// solver20.hip -- the matrix-free operator of HEX20 micro-element domains.
+//
+// A HEX20 micro mesh (gen_quadmesh_from_phi with 20-node elements,
+// f-src/mod_gen_quadmesh.f90:244-265) is solved with the same batched PCG,
+// reductions and FP64 refinement as the HEX08 voxel lattice; only the operator
+// and the node-indexed auxiliaries differ:
+//  * solver nodes are the mesh nodes in mesh order (node_begin + mesh id - 1);
+//    consecutive runs of 128 nodes form the pseudo-bricks that the vector
+//    kernels and the per-brick partial sums work on;
+//  * y = K x is a gather per node: for every (element, local node a) that
+//    touches the node, the three rows 3a..3a+2 of the 60x60 element stiffness
+//    (E = 1, in LDS) times the element's 60 gathered values, scaled by the
+//    element modulus.  Adjacency lists are sorted, so sums are deterministic;
+//  * Dirichlet nodes are the mesh nodes on the domain box faces
+//    (f-src/mod_struct_preprocess.f90:564-592), Jacobi uses diag(K) per DOF.
+#include <algorithm>
+#include <cmath>
+
+#include "bc.hpp"
+#include "solver.hpp"
+
+namespace dtc {
+
+namespace {
+
+constexpr int R = R24;
+constexpr int NT20 = 192;        // 8 nodes x 24 load cases per pass
+constexpr int PB = 128;          // nodes per pseudo-brick
+constexpr int FFD = 60;          // macro DOF slots of the force partials
+
+// element -> voxel modulus (kept elements only)
+__global__ void k20_emod(const int* __restrict__ keepE, const int* __restrict__ newE, const double* __restrict__ modulus,
+                         double E, double escale, int64_t nvox, double* __restrict__ emod) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox || !keepE[v]) return;
+    emod[newE[v]] = (modulus ? modulus[v] : E) * escale;
+}
+
+// 1-based int64 connectivity -> 0-based int32 local node ids (-1: node dropped)
+__global__ void k20_elems(const int64_t* __restrict__ e64, int64_t n, int* __restrict__ e32) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) e32[i] = (int)e64[i] - 1;
+}
+
+__global__ void k20_bc(const double* __restrict__ xyz, int64_t nn, DomInfo D, const double* __restrict__ dim,
+                       unsigned char* __restrict__ bcf, int* __restrict__ count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const int f = on_box_face(xyz + 3 * i, D.min_c, D.max_c, dim) ? 1 : 0;
+    bcf[i] = (unsigned char)f;
+    if (f) atomicAdd(count, 1);
+}
+
+__global__ void k20_adj_count(const int* __restrict__ elems, int64_t ne, int* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne * 20) return;
+    const int q = elems[i];
+    if (q >= 0) atomicAdd(&cnt[q], 1);
+}
+
+__global__ void k20_adj_fill(const int* __restrict__ elems, int64_t ne, const int* __restrict__ off,
+                             int* __restrict__ cursor, int* __restrict__ adj) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne * 20) return;
+    const int q = elems[i];
+    if (q < 0) return;
+    const int e = (int)(i / 20), a = (int)(i % 20);
+    adj[off[q] + atomicAdd(&cursor[q], 1)] = (e << 5) | a;
+}
+
+// sort each node's (element, local node) list: fixed summation order
+__global__ void k20_adj_sort(const int* __restrict__ off, int64_t nn, int* __restrict__ adj) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nn) return;
+    const int b = off[q], e = off[q + 1];
+    for (int i = b + 1; i < e; ++i) {
+        const int v = adj[i];
+        int j = i - 1;
+        while (j >= b && adj[j] > v) { adj[j + 1] = adj[j]; --j; }
+        adj[j + 1] = v;
+    }
+}
+
+// Jacobi: diag(K) per DOF from the node's elements; 0 on Dirichlet DOFs, 1 for a zero diagonal
+__global__ void k20_dinv(Dom20Dev G, const double* __restrict__ K, int64_t node0, double* __restrict__ d64,
+                         float* __restrict__ d32) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= G.n_nodes) return;
+    for (int d = 0; d < 3; ++d) {
+        double s = 0.0;
+        for (int k = G.adj_off[q]; k < G.adj_off[q + 1]; ++k) {
+            const int e = G.adj[k] >> 5, a = G.adj[k] & 31;
+            s += G.emod[e] * K[(3 * a + d) * 60 + 3 * a + d];
+        }
+        double inv = s != 0.0 ? 1.0 / s : 1.0;
+        if (G.bcf[q]) inv = 0.0;
+        d64[3 * (node0 + q) + d] = inv;
+        d32[3 * (node0 + q) + d] = (float)inv;
+    }
+}
+
+// boundary value of node ql (local) for load case lc of the domain's group
+__device__ inline double bc20(const DomInfo& D, const Dom20Dev& G, int ql, int d, int lc) {
+    double N[20];
+    macro_shape(D.macro_order, G.coords + 3 * ql, D.min_c, D.max_c, N);
+    return bc_value_m(D.macro_order, N, d, D.lc_base + lc);
+}
+
+struct Mv20 {
+    const BrickInfo* bricks;
+    const DomInfo* doms;
+    const Dom20Dev* d20;
+    const int* done;
+    double* partial;
+};
+
+// y = K x on the pseudo-bricks; modes as the HEX08 kernel (IN_V / IN_U input,
+// MASKED_DOT / RESID / PLAIN / MASKED output)
+template <typename T, int IN, int OUT>
+__global__ __launch_bounds__(NT20) void k_matvec20(Mv20 A, const T* __restrict__ Kg, const T* __restrict__ xin,
+                                                   T* __restrict__ yout) {
+    __shared__ T sK[3600];
+    __shared__ double s_red[NT20];
+    const BrickInfo B = A.bricks[blockIdx.x];
+    if (A.done && A.done[B.dom]) return;
+    for (int i = threadIdx.x; i < 3600; i += NT20) sK[i] = Kg[i];
+    __syncthreads();
+    const DomInfo D = A.doms[B.dom];
+    const Dom20Dev G = A.d20[B.dom];
+    const int tid = threadIdx.x, r = tid % R, g = tid / R;
+    double acc = 0.0;
+    for (int q = B.node_begin + g; q < B.node_end; q += NT20 / R) {
+        const int ql = (int)(q - G.node_begin);
+        T y[3] = {T(0), T(0), T(0)};
+        for (int k = G.adj_off[ql]; k < G.adj_off[ql + 1]; ++k) {
+            const int e = G.adj[k] >> 5, a = G.adj[k] & 31;
+            const int* en = G.elems + (int64_t)e * 20;
+            const T* Ka = sK + 3 * a * 60;
+            T s[3] = {T(0), T(0), T(0)};
+            for (int bb = 0; bb < 20; ++bb) {
+                const int qb = en[bb];
+                if (qb < 0) continue;
+                T xv[3];
+#pragma unroll
+                for (int d2 = 0; d2 < 3; ++d2) {
+                    if (IN == 1 && G.bcf[qb]) xv[d2] = (T)bc20(D, G, qb, d2, r);
+                    else if (IN == 1 && !xin) xv[d2] = T(0);
+                    else xv[d2] = xin[((int64_t)(G.node_begin + qb) * 3 + d2) * R + r];
+                }
+#pragma unroll
+                for (int dd = 0; dd < 3; ++dd)
+#pragma unroll
+                    for (int d2 = 0; d2 < 3; ++d2) s[dd] += Ka[dd * 60 + 3 * bb + d2] * xv[d2];
+            }
+            const T E = (T)G.emod[e];
+#pragma unroll
+            for (int dd = 0; dd < 3; ++dd) y[dd] += E * s[dd];
+        }
+        const bool bc = G.bcf[ql] != 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int64_t o = ((int64_t)q * 3 + d) * R + r;
+            T v = y[d];
+            if (OUT == 0) {          // masked, x . y
+                if (bc) v = T(0);
+                acc += (double)xin[o] * (double)v;
+            } else if (OUT == 1) {   // -y on free rows, y . y
+                v = bc ? T(0) : -v;
+                acc += (double)v * (double)v;
+            } else if (OUT == 3) {
+                if (bc) v = T(0);
+            }
+            yout[o] = v;
+        }
+    }
+    if (OUT == 0 || OUT == 1) {
+        s_red[tid] = acc;
+        __syncthreads();
+        if (tid < R) {
+            double sum = 0.0;
+            for (int gg = 0; gg < NT20 / R; ++gg) sum += s_red[gg * R + tid];
+            A.partial[(int64_t)blockIdx.x * R + tid] = sum;
+        }
+    }
+}
+
+// force projection over the domain's mesh nodes (HEX20 micro), same partial
+// layout as the lattice kernel: [brick][24 lc][FFD]
+template <int NEN>
+__global__ __launch_bounds__(NT20) void k_project20(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
+                                                    const Dom20Dev* __restrict__ d20, const double* __restrict__ rf,
+                                                    double* __restrict__ pff) {
+    constexpr int G8 = NT20 / R, FD = 3 * NEN, PASS = 12;
+    __shared__ double s[G8][R][PASS];
+    const BrickInfo B = bricks[blockIdx.x];
+    const DomInfo D = doms[B.dom];
+    if (macro_nodes(D.macro_order) != NEN) return;
+    const Dom20Dev G = d20[B.dom];
+    const int tid = threadIdx.x, lc = tid % R, g = tid / R;
+    double acc[FD];
+#pragma unroll
+    for (int k = 0; k < FD; ++k) acc[k] = 0.0;
+    for (int q = B.node_begin + g; q < B.node_end; q += G8) {
+        const int ql = (int)(q - G.node_begin);
+        double N[20];
+        macro_shape(D.macro_order, G.coords + 3 * ql, D.min_c, D.max_c, N);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double rv = rf[((int64_t)q * 3 + d) * R + lc];
+#pragma unroll
+            for (int l = 0; l < NEN; ++l) acc[d * NEN + l] += rv * N[l];
+        }
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < FD; k0 += PASS) {
+#pragma unroll
+        for (int k = 0; k < PASS; ++k) s[g][lc][k] = k0 + k < FD ? acc[k0 + k] : 0.0;
+        __syncthreads();
+        for (int i = tid; i < R * PASS; i += NT20) {
+            const int l2 = i / PASS, k = i % PASS;
+            if (k0 + k >= FD) continue;
+            double v = 0.0;
+            for (int gg = 0; gg < G8; ++gg) v += s[gg][l2][k];
+            pff[((int64_t)blockIdx.x * R + l2) * FFD + k0 + k] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// u = x + x_bc and r of domain dom in mesh numbering, column-major (3N, ncol)
+__global__ __launch_bounds__(NT20) void k_fields20(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
+                                                   const Dom20Dev* __restrict__ d20, const double* __restrict__ x,
+                                                   const double* __restrict__ rf, int dom, int ncol,
+                                                   double* __restrict__ uo, double* __restrict__ ro) {
+    const BrickInfo B = bricks[blockIdx.x];
+    if (B.dom != dom) return;
+    const DomInfo D = doms[B.dom];
+    const Dom20Dev G = d20[B.dom];
+    const int64_t nn = G.n_nodes;
+    for (int i = threadIdx.x; i < (B.node_end - B.node_begin) * 3 * R; i += NT20) {
+        const int o = i / (3 * R), rem = i % (3 * R), d = rem / R, lc = rem % R;
+        if (lc >= ncol) continue;
+        const int64_t q = B.node_begin + o;
+        const int ql = (int)(q - G.node_begin);
+        const int64_t out = (3 * (int64_t)ql + d) + 3 * nn * lc;
+        const int64_t idx = (q * 3 + d) * R + lc;
+        if (uo) uo[out] = G.bcf[ql] ? bc20(D, G, ql, d, lc) : x[idx];
+        if (ro) ro[out] = rf[idx];
+    }
+}
+
+inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <typename T, int IN, int OUT>
+int mv20_launch(SolverBatch& b, const void* xin, void* yout, double* part, const int* done) {
+    if (b.n_bricks == 0) return DTC_OK;
+    Mv20 a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.d20dev.as<Dom20Dev>(), done, part};
+    const T* K = sizeof(T) == 8 ? (const T*)b.k20_64.p : (const T*)b.k20_32.p;
+    k_matvec20<T, IN, OUT><<<b.n_bricks, NT20, 0, b.stream>>>(a, K, (const T*)xin, (T*)yout);
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+}  // namespace
+
+int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout, double* part, const int* done) {
+    if (fp64) {
+        if (in == 0 && out == 0) return mv20_launch<double, 0, 0>(b, xin, yout, part, done);
+        if (in == 1 && out == 1) return mv20_launch<double, 1, 1>(b, xin, yout, part, done);
+        if (in == 1 && out == 2) return mv20_launch<double, 1, 2>(b, xin, yout, part, done);
+    } else {
+        if (in == 0 && out == 0) return mv20_launch<float, 0, 0>(b, xin, yout, part, done);
+        if (in == 0 && out == 3) return mv20_launch<float, 0, 3>(b, xin, yout, part, done);
+    }
+    return set_error(DTC_ERR_UNSUPPORTED, "HEX20 micro operator: mode (%d, %d)", in, out);
+}
+
+int batch_add_domain20(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d, const double* modulus_dev,
+                       double escale, int lc_base, hipStream_t s) {
+    if (m.nen != 20) return set_error(DTC_ERR_BAD_ARGUMENT, "batch_add_domain20 needs a HEX20 mesh");
+    DomInfo D{};
+    D.nx = m.nx; D.ny = m.ny; D.nz = m.nz;
+    D.macro_order = d.macro_order;
+    D.lc_base = lc_base;
+    for (int k = 0; k < 3; ++k) {
+        D.delta[k] = d.delta[k];
+        D.xa1[k] = (double)(d.xa_n[k] - 1);
+        D.min_c[k] = (double)(d.xa_n[k] - 1) * d.delta[k];
+        D.max_c[k] = (double)(d.xa_n[k] + d.n[k] - 1) * d.delta[k];
+    }
+    const double dim[3] = {d.n[0] * d.delta[0], d.n[1] * d.delta[1], d.n[2] * d.delta[2]};
+    const int64_t nn = m.n_nodes, ne = m.n_elems;
+    if (b.n_nodes + nn >= INT32_MAX / 72 || ne >= (1 << 26))
+        return set_error(DTC_ERR_BAD_ARGUMENT, "HEX20 batch too large (%lld nodes, %lld elements)", (long long)nn,
+                         (long long)ne);
+    Dom20 G;
+    DevMem e64, dimd, cnt, cursor;
+    DTC_TRY(G.coords.alloc((size_t)nn * 3 * 8));
+    DTC_TRY(e64.alloc((size_t)ne * 20 * 8));
+    DTC_TRY(mesh_write_outputs(m, d.delta, d.xa_n, G.coords.as<double>(), e64.as<int64_t>(), s));
+    DTC_TRY(G.elems.alloc((size_t)ne * 20 * 4));
+    k20_elems<<<nb(ne * 20, 256), 256, 0, s>>>(e64.as<int64_t>(), ne * 20, G.elems.as<int>());
+    DTC_TRY(G.emod.alloc((size_t)ne * 8));
+    const int64_t nvox = (int64_t)m.nx * m.ny * m.nz;
+    k20_emod<<<nb(nvox, 256), 256, 0, s>>>(m.keepE.as<int>(), m.newE.as<int>(), modulus_dev, d.E, escale, nvox,
+                                          G.emod.as<double>());
+    // boundary nodes (fewer than 6 constrained DOFs fails the domain, f-src/mod_struct_preprocess.f90:471-473)
+    DTC_TRY(G.bcf.alloc((size_t)nn));
+    DTC_TRY(dimd.alloc(3 * 8));
+    DTC_TRY(cnt.alloc((size_t)(nn + 1) * 4));
+    DTC_TRY(cursor.alloc((size_t)(nn + 1) * 4));
+    DTC_HIP(hipMemcpyAsync(dimd.p, dim, 3 * 8, hipMemcpyHostToDevice, s));
+    DTC_HIP(hipMemsetAsync(cnt.p, 0, (size_t)(nn + 1) * 4, s));
+    DTC_HIP(hipMemsetAsync(cursor.p, 0, (size_t)(nn + 1) * 4, s));
+    k20_bc<<<nb(nn, 256), 256, 0, s>>>(G.coords.as<double>(), nn, D, dimd.as<double>(), G.bcf.as<unsigned char>(),
+                                       cnt.as<int>() + nn);
+    int hb = 0;
+    DTC_HIP(hipMemcpyAsync(&hb, cnt.as<int>() + nn, 4, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipStreamSynchronize(s));
+    if (3 * hb < 6) {
+        DTC_TRY(batch_add_empty(b, d, DTC_ERR_FEW_CONSTRAINTS, lc_base));
+        b.h_elems.back() = ne;
+        b.h_nodes.back() = nn;
+        b.h_bnodes.back() = hb;
+        return DTC_OK;
+    }
+    // node -> (element, local node) adjacency, sorted per node
+    DTC_HIP(hipMemsetAsync(cnt.p, 0, (size_t)(nn + 1) * 4, s));
+    k20_adj_count<<<nb(ne * 20, 256), 256, 0, s>>>(G.elems.as<int>(), ne, cnt.as<int>());
+    DTC_TRY(G.adj_off.alloc((size_t)(nn + 1) * 4));
+    int64_t total = 0;
+    DTC_TRY(scan_exclusive_i32(cnt.as<int>(), G.adj_off.as<int>(), nn + 1, &total, G.scan_tmp, s));
+    DTC_TRY(G.adj.alloc((size_t)std::max<int64_t>(total, 1) * 4));
+    k20_adj_fill<<<nb(ne * 20, 256), 256, 0, s>>>(G.elems.as<int>(), ne, G.adj_off.as<int>(), cursor.as<int>(),
+                                                  G.adj.as<int>());
+    k20_adj_sort<<<nb(nn, 256), 256, 0, s>>>(G.adj_off.as<int>(), nn, G.adj.as<int>());
+    DTC_HIP(hipGetLastError());
+    // pseudo-bricks of PB consecutive nodes
+    D.brick_begin = (int)b.h_brick.size();
+    D.node_begin = b.n_nodes;
+    for (int64_t k = 0; k < nn; k += PB) {
+        BrickInfo B{};
+        B.dom = b.n_dom;
+        B.node_begin = (int)(b.n_nodes + k);
+        B.node_end = (int)(b.n_nodes + std::min<int64_t>(nn, k + PB));
+        b.h_brick.push_back(B);
+    }
+    D.brick_end = (int)b.h_brick.size();
+    D.node_end = b.n_nodes + nn;
+    D.lat_off = b.n_lat;
+    D.vox_off = b.n_vox;
+    G.node_begin = b.n_nodes;
+    G.n_nodes = nn;
+    DTC_HIP(hipStreamSynchronize(s));   // temporaries go out of scope
+    b.d20.push_back(std::move(G));
+    b.h_dom.push_back(D);
+    b.n_nodes += nn;
+    b.n_lat += (int64_t)(m.nx + 1) * (m.ny + 1) * (m.nz + 1);
+    b.n_vox += nvox;
+    b.n_dom += 1;
+    b.h_elems.push_back(ne);
+    b.h_nodes.push_back(nn);
+    b.h_bnodes.push_back(hb);
+    b.h_status.push_back(DTC_OK);
+    return DTC_OK;
+}
+
+int finalize20(SolverBatch& b) {
+    hipStream_t s = b.stream;
+    if (b.khat20.size() != 3600) return set_error(DTC_ERR_BAD_ARGUMENT, "HEX20 element stiffness missing");
+    std::vector<Dom20Dev> h(b.n_dom);
+    for (int i = 0; i < b.n_dom; ++i) {
+        const Dom20& G = b.d20[i];
+        h[i] = Dom20Dev{G.coords.as<double>(), G.elems.as<int>(), G.emod.as<double>(), G.adj_off.as<int>(),
+                        G.adj.as<int>(), G.bcf.as<unsigned char>(), G.node_begin, (int)G.n_nodes};
+    }
+    DTC_TRY(b.d20dev.ensure(std::max<size_t>(h.size(), 1) * sizeof(Dom20Dev)));
+    if (!h.empty()) DTC_HIP(hipMemcpyAsync(b.d20dev.p, h.data(), h.size() * sizeof(Dom20Dev), hipMemcpyHostToDevice, s));
+    std::vector<float> k32(3600);
+    for (int i = 0; i < 3600; ++i) k32[i] = (float)b.khat20[i];
+    DTC_TRY(b.k20_64.ensure(3600 * 8));
+    DTC_TRY(b.k20_32.ensure(3600 * 4));
+    DTC_HIP(hipMemcpyAsync(b.k20_64.p, b.khat20.data(), 3600 * 8, hipMemcpyHostToDevice, s));
+    DTC_HIP(hipMemcpyAsync(b.k20_32.p, k32.data(), 3600 * 4, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < b.n_dom; ++i)
+        if (h[i].n_nodes > 0)
+            k20_dinv<<<nb(h[i].n_nodes, 256), 256, 0, s>>>(h[i], b.k20_64.as<double>(), h[i].node_begin,
+                                                           b.dinv64.as<double>(), b.dinv32.as<float>());
+    DTC_HIP(hipGetLastError());
+    DTC_HIP(hipStreamSynchronize(s));
+    return DTC_OK;
+}
+
+int project20(SolverBatch& b) {
+    if (b.n_bricks == 0) return DTC_OK;
+    k_project20<8><<<b.n_bricks, NT20, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
+                                                      b.d20dev.as<Dom20Dev>(), b.r64.as<double>(), b.part_ff.as<double>());
+    k_project20<20><<<b.n_bricks, NT20, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
+                                                       b.d20dev.as<Dom20Dev>(), b.r64.as<double>(),
+                                                       b.part_ff.as<double>());
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+int fields20(SolverBatch& b, int i, int ncol, double* u_dev, double* r_dev) {
+    k_fields20<<<b.n_bricks, NT20, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.d20dev.as<Dom20Dev>(),
+                                                  b.x64.as<double>(), b.r64.as<double>(), i, ncol, u_dev, r_dev);
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+}  // namespace dtc

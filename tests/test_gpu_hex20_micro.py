@@ -1,0 +1,113 @@
+"""GPU parity of HEX20 micro-element domains (node-gather operator) against the
+oracle: closed forms, porous domains, both macro elements, failures in a batch.
+Bars as for HEX08: relative 1e-6 (FP64) / 1e-5 (FP32 + FP64 refinement)."""
+import numpy as np
+import pytest
+
+from helpers import iso_voigt
+
+pytestmark = pytest.mark.gpu
+
+
+def _dtc():
+    import dtc_amd
+    return dtc_amd
+
+
+def _O():
+    from oracle import oracle as O
+    return O
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _oracle(d, macro_order=1, want=False):
+    O = _O()
+    nodes, elems = O.mesh(d.phi, d.llimit, 20, d.delta, d.xa_n)
+    mn, mx, dim = d.box()
+    r = O.solve_domain(nodes, elems, d.E, d.nu, d.a(), mn, mx, dim, tol=1e-12, want_u=want, want_r=want,
+                       macro_order=macro_order)
+    KM, Cm, EE = O.effective(r["ff"], dim, d.rve_strain, macro_order=macro_order)
+    return dict(r, KM=KM, C=Cm, EE=EE, nodes=nodes, elems=elems)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-5)])
+def test_hex20_micro_solid_cube_closed_form(prec, tol):
+    dtc = _dtc()
+    d = dtc.Domain(np.ones((5, 5, 5), np.int8), delta=(0.02,) * 3, xa_n=(3, 3, 3), micro_nen=20)
+    r = dtc.solve_domain(d, precision=prec, rel_tol=1e-8, rotation=False)
+    assert r["status"] == 0 and r["relres"] <= 1e-8
+    assert r["n_elems"] == 125 and r["n_nodes"] == 6 ** 3 + 3 * 5 * 36
+    assert _rel(r["C"], iso_voigt(5600.0, 0.3)) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-5)])
+def test_hex20_micro_porous_domain_matches_oracle(prec, tol):
+    dtc = _dtc()
+    from dtc_amd import synth
+    phi = synth.trabecular(9, seed=81, sigma=1.4, bvtv=0.45)
+    d = dtc.Domain(phi, delta=(0.01,) * 3, xa_n=(2, 2, 2), micro_nen=20)
+    r = dtc.solve_domain(d, precision=prec, rel_tol=1e-8, rotation=False, want_u=True, want_r=True)
+    o = _oracle(d, want=True)
+    assert r["status"] == 0 and r["relres"] <= 1e-8
+    assert r["n_elems"] == o["elems"].shape[0] and r["n_nodes"] == o["nodes"].shape[0]
+    for key in ("ff", "K_macro", "C", "EE"):
+        ref = o["KM"] if key == "K_macro" else o[key]
+        assert _rel(r[key], ref) < tol, key
+    assert r["u"].shape == o["u"].shape == (24, 3 * o["nodes"].shape[0])
+    assert _rel(r["u"], o["u"]) < 10 * tol
+    assert np.abs(r["r"] - o["r"]).max() < 10 * tol * np.abs(o["r"]).max()
+
+
+def test_hex20_micro_with_hex20_macro_matches_oracle():
+    dtc = _dtc()
+    from dtc_amd import synth
+    phi = synth.trabecular(8, seed=82, sigma=1.4, bvtv=0.5)
+    d = dtc.Domain(phi, delta=(0.01,) * 3, micro_nen=20, macro_order=2)
+    r = dtc.solve_domain(d, precision=1, rel_tol=1e-8, rotation=False)
+    o = _oracle(d, macro_order=2)
+    assert r["status"] == 0 and r["ff"].shape == (60, 60)
+    assert _rel(r["ff"], o["ff"]) < 1e-5
+    assert _rel(r["C"], o["C"]) < 1e-5 and _rel(r["EE"], o["EE"]) < 1e-5
+
+
+def test_hex20_micro_batch_with_failures_and_per_voxel_modulus():
+    dtc = _dtc()
+    from dtc_amd import synth
+    O = _O()
+    grey, modulus = synth.greyscale(8, seed=6, bvtv=0.5)
+    d1 = dtc.Domain(grey, llimit=1, delta=(0.01,) * 3, modulus=modulus, E=10000.0, micro_nen=20)
+    d2 = dtc.Domain(np.zeros((4, 4, 4), np.int8), micro_nen=20)                 # empty
+    few = np.zeros((5, 5, 5), np.int8); few[2, 2, 2] = 1                          # no node on the box
+    d3 = dtc.Domain(few, micro_nen=20)
+    phi = synth.trabecular(7, seed=83, sigma=1.3, bvtv=0.5)
+    d4 = dtc.Domain(phi, delta=(0.02,) * 3, xa_n=(3, 4, 5), micro_nen=20)
+    rb = dtc.solve_batch([d1, d2, d3, d4], precision=1, rel_tol=1e-8, rotation=True, precond=1)
+    assert [x["status"] for x in rb] == [0, 1, 2, 0]
+    # per-voxel modulus: element e of the mesh takes the modulus of its voxel
+    nodes, elems = O.mesh(grey, 1, 20, d1.delta, d1.xa_n)
+    base = nodes[elems[:, 0] - 1]
+    ijk = np.rint(base / np.array(d1.delta) - (np.array(d1.xa_n) - 1)).astype(int)
+    mn, mx, dim = d1.box()
+    o1 = O.solve_domain(nodes, elems, d1.E, d1.nu, d1.a(), mn, mx, dim, tol=1e-12,
+                        emod=modulus[ijk[:, 2], ijk[:, 1], ijk[:, 0]])
+    _, C1, _ = O.effective(o1["ff"], dim)
+    assert _rel(rb[0]["C"], C1) < 1e-5
+    assert _rel(rb[3]["C"], _oracle(d4)["C"]) < 1e-5
+    s4 = dtc.solve_domain(d4, precision=1, rel_tol=1e-8, rotation=True)
+    assert _rel(rb[3]["C"], s4["C"]) < 1e-6
+    for c in range(2):
+        assert rb[3]["rot"][c]["alpha"] == pytest.approx(s4["rot"][c]["alpha"], abs=1e-6)
+
+
+def test_hex20_micro_rejects_mixed_element_types_and_gradient():
+    dtc = _dtc()
+    phi = np.ones((4, 4, 4), np.int8)
+    with pytest.raises(Exception):
+        dtc.solve_batch([dtc.Domain(phi), dtc.Domain(phi, micro_nen=20)], precision=1)
+    with pytest.raises(Exception):
+        with dtc.Batch([dtc.Domain(phi, micro_nen=20)], precision=1) as b:
+            b.run()
+            b.grad(0, np.eye(6))
