@@ -121,13 +121,40 @@ __device__ inline void walsh_apply(const WK<T>& k, const T w[8][3], T y[8][3]) {
 #undef M2
 }
 
+// ---------------------------------------------------------------- wave sums
+// FP64 sum over the 64 lanes of a wave with DPP lane moves (VALU, no LDS round
+// trip): lanes xor 1 and xor 2 inside quads, then the half-row and row mirrors
+// leave every lane of a 16-lane row with the row sum (each step adds a value
+// and its mirror image, so all lanes hold the same bits); the four row sums are
+// added in row order.  Fixed order: deterministic.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double wave_sum64(double v) {
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);   // row_half_mirror
+    v += dpp_f64<0x140>(v);   // row_mirror
+    return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
+}
+
 // ---------------------------------------------------------------- matvec
 // Load cases are processed in chunks of RC: one (node, direction) segment of a
 // chunk is one 16-byte vector V (float4 / double2).
 template <typename T> struct MvCfg;
 template <> struct MvCfg<float> { static constexpr int RC = 4; using V = float4; };
 template <> struct MvCfg<double> { static constexpr int RC = 2; using V = double2; };
-constexpr int MV_NT = 256;
+constexpr int MV_NT = 512;
 constexpr int NH = HX * HY * HZ, NET = EX * EY * EZ, NO = BX * BY * BZ;
 constexpr int OB_BIAS = 1 + BX + BX * BY;   // owned-layout base of tile element (0,0,0) is -OB_BIAS
 
@@ -164,8 +191,10 @@ struct MvArgs {
     float c1, c2;               // Chebyshev coefficients of this step
 };
 
-// y = K x for every brick of the batch.  One workgroup per brick and chunk
-// loop over the load cases:
+// y = K x for every brick of the batch.  One workgroup per (brick, chunk
+// range): blockIdx.y splits the NCH chunks into gridDim.y equal ranges, so that
+// small grids (coarse multigrid levels, single domains) still fill the chip.
+// Chunk loop over the load cases:
 //   stage   the halo of x (10x6x6 nodes) goes to LDS (16-byte loads, issued
 //           one chunk ahead into registers);
 //   product every solid element of the 9x5x5 tile is one task per load case
@@ -202,9 +231,9 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
     const int* __restrict__ halo = A.halo + (int64_t)blockIdx.x * NH;
     const V* xv = reinterpret_cast<const V*>(xin);
     V* yv = reinterpret_cast<V*>(yout);
+    const int c0 = (int)blockIdx.y * NCH / (int)gridDim.y, c1 = ((int)blockIdx.y + 1) * NCH / (int)gridDim.y;
 
-    // x loads run two chunks ahead: chunk c is staged from regs[c % 2]
-    static_assert(NCH % 2 == 0, "chunk loop is unrolled by two");
+    // x loads run two chunks ahead: chunk c is staged from regA (c - c0 even) or regB
     V regA[NLD], regB[NLD];
     // chunk-invariant 16-byte offsets of the halo segments this thread stages
     int pf_off[NLD];
@@ -228,13 +257,17 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
         }
     };
     if (IN == IN_V) {
-        prefetch(regA, 0);
-        prefetch(regB, 1);
+        prefetch(regA, c0);
+        if (c0 + 1 < c1) prefetch(regB, c0 + 1);
     }
     // chunk-invariant gather items of this thread: owned (node, direction) i = tid + j * MV_NT
     constexpr int NG = (NO * 3 + MV_NT - 1) / MV_NT;
     int g_off[NG], g_xi[NG];
     unsigned g_m[NG];
+    // OUT_BRES / OUT_CHEB operands of the gather items, loaded before the element
+    // phase so that their latency overlaps it; the Jacobi scaling is chunk-invariant
+    V e_b[NG], e_d[NG];
+    T e_di[NG];
     for (int i = tid; i < NH; i += MV_NT) s_node[i] = halo[i];
     if (tid < NO) s_nm[tid] = A.nmask[(int64_t)blockIdx.x * NO + tid];
     const int ne = B.e_cnt;
@@ -278,8 +311,8 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             }
         }
         __syncthreads();
-        if (IN == IN_V && chunk + 2 < NCH) prefetch(reg, chunk + 2);
-        if (chunk == 0) {
+        if (IN == IN_V && chunk + 2 < c1) prefetch(reg, chunk + 2);
+        if (chunk == c0) {
 #pragma unroll
             for (int j = 0; j < NG; ++j) {
                 const int i = tid + j * MV_NT;
@@ -297,6 +330,21 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                 g_off[j] = off;
                 g_xi[j] = xi;
                 g_m[j] = m;
+                e_di[j] = T(0);
+                if (OUT == OUT_CHEB && off >= 0 && !(m & 256u))
+                    e_di[j] = reinterpret_cast<const T*>(A.dinv)[off / NCH];
+            }
+        }
+        if (OUT == OUT_BRES || OUT == OUT_CHEB) {
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                vzero(e_b[j]);
+                vzero(e_d[j]);
+                if (g_off[j] >= 0) {
+                    const int vi = g_off[j] + chunk;
+                    e_b[j] = reinterpret_cast<const V*>(A.bvec)[vi];
+                    if (OUT == OUT_CHEB) e_d[j] = reinterpret_cast<const V*>(A.dvec)[vi];
+                }
             }
         }
 
@@ -370,7 +418,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             } else if (OUT == OUT_MASKED) {
                 if (bc) vzero(y);
             } else if (OUT == OUT_BRES || OUT == OUT_CHEB) {
-                const V bv = reinterpret_cast<const V*>(A.bvec)[vi];
+                const V bv = e_b[j];
                 if (OUT == OUT_BRES) {
                     if (bc) vzero(y);
                     else {
@@ -384,8 +432,8 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                     vzero(dn);
                     vzero(xn);
                     if (!bc) {
-                        const T di = reinterpret_cast<const T*>(A.dinv)[g_off[j] / NCH];
-                        const V dv = *dp, xv = s_x[g_xi[j]];
+                        const T di = e_di[j];
+                        const V dv = e_d[j], xv = s_x[g_xi[j]];
                         T* dnp = reinterpret_cast<T*>(&dn);
                         T* xnp = reinterpret_cast<T*>(&xn);
 #pragma unroll
@@ -405,25 +453,21 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             yv[vi] = y;
         }
         if (OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) {
-            // every lane holds the RC load cases of the chunk: fixed butterfly over the wave
+            // every lane holds the RC load cases of the chunk: fixed-order wave sums
 #pragma unroll
-            for (int w = 0; w < RC; ++w) {
-                double v = acc[w];
-#pragma unroll
-                for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m);
-                acc[w] = v;
-            }
+            for (int w = 0; w < RC; ++w) acc[w] = wave_sum64(acc[w]);
             if (tid % 64 == 0)
 #pragma unroll
                 for (int w = 0; w < RC; ++w) s_wred[chunk][tid / 64][w] = acc[w];
         }
         __syncthreads();
     };
-    for (int chunk = 0; chunk < NCH; chunk += 2) {
+    for (int chunk = c0; chunk < c1; chunk += 2) {
         chunk_body(chunk, regA);
-        chunk_body(chunk + 1, regB);
+        if (chunk + 1 < c1) chunk_body(chunk + 1, regB);
     }
-    if ((OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) && tid < R) {
+    if ((OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) && tid < R &&
+        tid / RC >= c0 && tid / RC < c1) {
         const int ch = tid / RC, r = tid % RC;
         double s = 0.0;
 #pragma unroll
@@ -1127,7 +1171,7 @@ __device__ inline void f4_axpy(float4& s, float w, const float4& a) {
 }
 
 // b_c = P^T r_f: trilinear weights over the free (interior) fine nodes; one
-// workgroup per coarse brick
+// workgroup per (coarse brick, blockIdx.y-th share of its nodes)
 __global__ __launch_bounds__(256) void k_restrict(const BrickInfo* __restrict__ cbr, const DomInfo* __restrict__ cdom,
                                                   const int* __restrict__ cl2n, const DomInfo* __restrict__ fdom,
                                                   const int* __restrict__ fl2n, const float* __restrict__ rf,
@@ -1142,7 +1186,7 @@ __global__ __launch_bounds__(256) void k_restrict(const BrickInfo* __restrict__ 
     const int fLx = F.nx + 1, fLy = F.ny + 1;
     const float4* rf4 = reinterpret_cast<const float4*>(rf);
     float4* bc4 = reinterpret_cast<float4*>(bc);
-    for (int o = slot; o < NO; o += T_NPP) {
+    for (int o = slot + (int)blockIdx.y * T_NPP; o < NO; o += T_NPP * (int)gridDim.y) {
         const int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
         if (gx > C.nx || gy > C.ny || gz > C.nz) continue;
         const int qc = cl2n[C.lat_off + gx + (int64_t)cLx * (gy + (int64_t)cLy * gz)];
@@ -1170,7 +1214,8 @@ __global__ __launch_bounds__(256) void k_restrict(const BrickInfo* __restrict__ 
     }
 }
 
-// x_f += P x_c on the free nodes of each fine brick (trilinear interpolation)
+// x_f += P x_c on the free nodes of each fine brick (trilinear interpolation);
+// blockIdx.y selects a share of the brick's nodes
 __global__ __launch_bounds__(256) void k_prolong(const BrickInfo* __restrict__ fbr, const DomInfo* __restrict__ fdom,
                                                  const int* __restrict__ fl2n, const DomInfo* __restrict__ cdom,
                                                  const int* __restrict__ cl2n, const float* __restrict__ xc,
@@ -1185,7 +1230,7 @@ __global__ __launch_bounds__(256) void k_prolong(const BrickInfo* __restrict__ f
     const int cLx = C.nx + 1, cLy = C.ny + 1;
     const float4* xc4 = reinterpret_cast<const float4*>(xc);
     float4* xf4 = reinterpret_cast<float4*>(xf);
-    for (int o = slot; o < NO; o += T_NPP) {
+    for (int o = slot + (int)blockIdx.y * T_NPP; o < NO; o += T_NPP * (int)gridDim.y) {
         const int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
         if (!(gx > 0 && gy > 0 && gz > 0 && gx < F.nx && gy < F.ny && gz < F.nz)) continue;   // free nodes only
         const int qf = fl2n[F.lat_off + gx + (int64_t)fLx * (gy + (int64_t)fLy * gz)];
@@ -1253,6 +1298,23 @@ __global__ __launch_bounds__(NT_UPD) void k_cheb(const BrickInfo* __restrict__ b
 }
 
 inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// Grids of fewer than FILL_BLOCKS workgroups leave CUs idle: the matvec then
+// splits each brick's load-case chunks over blockIdx.y (a divisor of nch), the
+// transfer kernels split each brick's nodes.
+constexpr int FILL_BLOCKS = 2048;
+inline int chunk_split(int n_bricks, int nch) {
+    int want = n_bricks > 0 ? (FILL_BLOCKS + n_bricks - 1) / n_bricks : 1;
+    if (want >= nch) return nch;
+    int best = 1;
+    for (int s = 1; s <= want; ++s)
+        if (nch % s == 0) best = s;
+    return best;
+}
+inline int node_split(int n_bricks) {
+    const int want = n_bricks > 0 ? (FILL_BLOCKS + n_bricks - 1) / n_bricks : 1;
+    return want < 1 ? 1 : want > 8 ? 8 : want;
+}
 
 }  // namespace
 
@@ -1742,7 +1804,8 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
     if (b.micro20) {
         DTC_TRY(mv20(b, sizeof(T) == 8, IN, OUT, xin, yout, part, done));
     } else {
-        k_matvec<T, IN, OUT><<<b.n_bricks, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
+        const dim3 grid(b.n_bricks, chunk_split(b.n_bricks, R / MvCfg<T>::RC));
+        k_matvec<T, IN, OUT><<<grid, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
         DTC_HIP(hipGetLastError());
     }
     if (t) DTC_HIP(hipEventRecord(e1, b.stream));
@@ -1766,7 +1829,8 @@ int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float
                 float c2 = 0.f, double* partial = nullptr) {
     if (T.n_bricks == 0) return DTC_OK;
     MvArgs a{T.bricks, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2};
-    k_matvec<float, IN_V, OUT><<<T.n_bricks, MV_NT, 0, s>>>(a, wk, x, y);
+    const dim3 grid(T.n_bricks, chunk_split(T.n_bricks, R / MvCfg<float>::RC));
+    k_matvec<float, IN_V, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
     DTC_HIP(hipGetLastError());
     return DTC_OK;
 }
@@ -1872,14 +1936,14 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
         if (c.T.n_bricks == 0 || f.T.n_bricks == 0) continue;
         float* res = f.buf[1 - cur[l]];
         DTC_TRY(launch_mv32<OUT_BRES>(f.T, s, wk, f.buf[cur[l]], res, done, f.b));
-        k_restrict<<<c.T.n_bricks, 256, 0, s>>>(c.T.bricks, c.T.doms, c.l2n, f.T.doms, f.l2n, res, done, c.b);
+        k_restrict<<<dim3(c.T.n_bricks, node_split(c.T.n_bricks)), 256, 0, s>>>(c.T.bricks, c.T.doms, c.l2n, f.T.doms, f.l2n, res, done, c.b);
         DTC_HIP(hipGetLastError());
     }
     DTC_TRY(smooth(L, P.coarse_degree, true, P.coarse_ratio, L == 0 ? prz : nullptr));
     for (int l = L - 1; l >= 0; --l) {
         const LvView f = view(l), c = view(l + 1);
         if (f.T.n_bricks > 0 && c.T.n_bricks > 0) {
-            k_prolong<<<f.T.n_bricks, 256, 0, s>>>(f.T.bricks, f.T.doms, f.l2n, c.T.doms, c.l2n, c.buf[cur[l + 1]], done,
+            k_prolong<<<dim3(f.T.n_bricks, node_split(f.T.n_bricks)), 256, 0, s>>>(f.T.bricks, f.T.doms, f.l2n, c.T.doms, c.l2n, c.buf[cur[l + 1]], done,
                                                    f.buf[cur[l]]);
             DTC_HIP(hipGetLastError());
         }
