@@ -233,8 +233,6 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
     V* yv = reinterpret_cast<V*>(yout);
     const int c0 = (int)blockIdx.y * NCH / (int)gridDim.y, c1 = ((int)blockIdx.y + 1) * NCH / (int)gridDim.y;
 
-    // x loads run two chunks ahead: chunk c is staged from regA (c - c0 even) or regB
-    V regA[NLD], regB[NLD];
     // chunk-invariant 16-byte offsets of the halo segments this thread stages
     int pf_off[NLD];
 #pragma unroll
@@ -247,7 +245,9 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
         }
         pf_off[k] = o;
     }
-    auto prefetch = [&](V (&reg)[NLD], int chunk) {
+    // x segments of the next chunk to stage, loaded one chunk ahead
+    V reg[NLD];
+    auto prefetch = [&](int chunk) {
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
             V v;
@@ -256,18 +256,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             reg[k] = v;
         }
     };
-    if (IN == IN_V) {
-        prefetch(regA, c0);
-        if (c0 + 1 < c1) prefetch(regB, c0 + 1);
-    }
-    // chunk-invariant gather items of this thread: owned (node, direction) i = tid + j * MV_NT
-    constexpr int NG = (NO * 3 + MV_NT - 1) / MV_NT;
-    int g_off[NG], g_xi[NG];
-    unsigned g_m[NG];
-    // OUT_BRES / OUT_CHEB operands of the gather items, loaded before the element
-    // phase so that their latency overlaps it; the Jacobi scaling is chunk-invariant
-    V e_b[NG], e_d[NG];
-    T e_di[NG];
+    if (IN == IN_V) prefetch(c0);
     for (int i = tid; i < NH; i += MV_NT) s_node[i] = halo[i];
     if (tid < NO) s_nm[tid] = A.nmask[(int64_t)blockIdx.x * NO + tid];
     const int ne = B.e_cnt;
@@ -279,9 +268,10 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             s_E[i] = eE[i];
         }
     }
-    if (IN != IN_V) __syncthreads();
+    __syncthreads();
 
-    auto chunk_body = [&](int chunk, V (&reg)[NLD]) {
+    // s_x <- x of `chunk` (IN_V: from the prefetch registers; IN_U: x + BC values)
+    auto stage = [&](int chunk) {
         if (IN == IN_V) {
 #pragma unroll
             for (int k = 0; k < NLD; ++k) {
@@ -310,41 +300,57 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                 sx[i] = v;
             }
         }
-        __syncthreads();
-        if (IN == IN_V && chunk + 2 < c1) prefetch(reg, chunk + 2);
-        if (chunk == c0) {
+    };
+
+    // chunk-invariant gather items of this thread: owned (node, direction) i = tid + j * MV_NT
+    constexpr int NG = (NO * 3 + MV_NT - 1) / MV_NT;
+    int g_off[NG], g_xi[NG];
+    unsigned g_m[NG];
+    T e_di[NG];   // OUT_CHEB: Jacobi scaling of the item
 #pragma unroll
-            for (int j = 0; j < NG; ++j) {
-                const int i = tid + j * MV_NT;
-                int off = -1, xi = 0;
-                unsigned m = 0;
-                if (i < NO * 3) {
-                    const int o = i / 3, d = i % 3, h = ohalo(o);
-                    const int q = s_node[h];
-                    if (q >= 0) {
-                        off = ((q & NODE_MASK) * 3 + d) * NCH;
-                        xi = h * 3 + d;
-                        m = (unsigned)s_nm[o] | ((q & BC_BIT) ? 256u : 0u) | ((unsigned)o << 9) | ((unsigned)d << 16);
-                    }
-                }
-                g_off[j] = off;
-                g_xi[j] = xi;
-                g_m[j] = m;
-                e_di[j] = T(0);
-                if (OUT == OUT_CHEB && off >= 0 && !(m & 256u))
-                    e_di[j] = reinterpret_cast<const T*>(A.dinv)[off / NCH];
+    for (int j = 0; j < NG; ++j) {
+        const int i = tid + j * MV_NT;
+        int off = -1, xi = 0;
+        unsigned m = 0;
+        if (i < NO * 3) {
+            const int o = i / 3, d = i % 3, h = ohalo(o);
+            const int q = s_node[h];
+            if (q >= 0) {
+                off = ((q & NODE_MASK) * 3 + d) * NCH;
+                xi = h * 3 + d;
+                m = (unsigned)s_nm[o] | ((q & BC_BIT) ? 256u : 0u) | ((unsigned)o << 9) | ((unsigned)d << 16);
             }
         }
-        if (OUT == OUT_BRES || OUT == OUT_CHEB) {
+        g_off[j] = off;
+        g_xi[j] = xi;
+        g_m[j] = m;
+        e_di[j] = T(0);
+        if (OUT == OUT_CHEB && off >= 0 && !(m & 256u)) e_di[j] = reinterpret_cast<const T*>(A.dinv)[off / NCH];
+    }
+
+    stage(c0);
+    if (IN == IN_V && c0 + 1 < c1) prefetch(c0 + 1);
+    __syncthreads();
+
+    // Per chunk: element products -> barrier -> stage the next chunk and issue
+    // the loads of the one after -> gather -> barrier.  The next chunk is staged
+    // before the gather, so the wait for its x loads (and for any earlier
+    // stores) sits behind a whole element phase instead of right behind the
+    // gather's stores.
+    for (int chunk = c0; chunk < c1; ++chunk) {
+        // gather operands of this chunk: b and d (OUT_BRES / OUT_CHEB) from memory,
+        // x from LDS before the next chunk overwrites it
+        V e_b[NG], e_d[NG], e_x[NG];
 #pragma unroll
-            for (int j = 0; j < NG; ++j) {
-                vzero(e_b[j]);
-                vzero(e_d[j]);
-                if (g_off[j] >= 0) {
-                    const int vi = g_off[j] + chunk;
-                    e_b[j] = reinterpret_cast<const V*>(A.bvec)[vi];
-                    if (OUT == OUT_CHEB) e_d[j] = reinterpret_cast<const V*>(A.dvec)[vi];
-                }
+        for (int j = 0; j < NG; ++j) {
+            vzero(e_b[j]);
+            vzero(e_d[j]);
+            vzero(e_x[j]);
+            if (g_off[j] >= 0) {
+                const int vi = g_off[j] + chunk;
+                if (OUT == OUT_BRES || OUT == OUT_CHEB) e_b[j] = reinterpret_cast<const V*>(A.bvec)[vi];
+                if (OUT == OUT_CHEB) e_d[j] = reinterpret_cast<const V*>(A.dvec)[vi];
+                if (OUT == OUT_MASKED_DOT || OUT == OUT_CHEB) e_x[j] = s_x[g_xi[j]];
             }
         }
 
@@ -388,6 +394,10 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             }
         }
         __syncthreads();
+        if (chunk + 1 < c1) {
+            stage(chunk + 1);
+            if (IN == IN_V && chunk + 2 < c1) prefetch(chunk + 2);
+        }
 
         // ---- gather: thread per owned (node, direction), all RC load cases in one vector
         double acc[RC];
@@ -408,9 +418,8 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             const int vi = g_off[j] + chunk;
             if (OUT == OUT_MASKED_DOT) {
                 if (bc) vzero(y);
-                const V x = s_x[g_xi[j]];
 #pragma unroll
-                for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(x, w) * (double)vcomp(y, w);
+                for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(e_x[j], w) * (double)vcomp(y, w);
             } else if (OUT == OUT_RESID) {
                 if (bc) vzero(y); else y = vneg(y);
 #pragma unroll
@@ -433,7 +442,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                     vzero(xn);
                     if (!bc) {
                         const T di = e_di[j];
-                        const V dv = e_d[j], xv = s_x[g_xi[j]];
+                        const V dv = e_d[j], xv = e_x[j];
                         T* dnp = reinterpret_cast<T*>(&dn);
                         T* xnp = reinterpret_cast<T*>(&xn);
 #pragma unroll
@@ -461,10 +470,6 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                 for (int w = 0; w < RC; ++w) s_wred[chunk][tid / 64][w] = acc[w];
         }
         __syncthreads();
-    };
-    for (int chunk = c0; chunk < c1; chunk += 2) {
-        chunk_body(chunk, regA);
-        if (chunk + 1 < c1) chunk_body(chunk + 1, regB);
     }
     if ((OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) && tid < R &&
         tid / RC >= c0 && tid / RC < c1) {
