@@ -126,6 +126,8 @@ def main():
         raise SystemExit(f"domains {bad} did not converge to 1e-8: {[res[i]['status'] for i in bad]}")
     value = len(doms) * args.steps / dt
     mixed_iters = np.mean([r["iterations"] for r in res])
+    log("PCG iterations per domain: " + " ".join(str(r["iterations"]) for r in res) +
+        "; FP64 residual replacements: " + " ".join(str(r["outer_iterations"]) for r in res))
 
     # ---- plain FP64 GPU form, one domain at a time
     t64, C64, it64 = 0.0, [], []

@@ -163,9 +163,11 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res);
  * after dtc_batch_run (self-adjoint: uses the stored load-case solutions,
  * no extra solve).  dL_dE: n^3 per voxel (0 where no element). */
 int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_dE);
-/* timing of the last run: average duration (ms) of the EbE matvec kernel
+/* timing of the last run: average duration (ms) of the CG matvec kernel
  * launches measured with HIP events on the batch's stream, the number of
- * timed launches, and the algorithmic bytes and flops of one launch. */
+ * timed launches, and the algorithmic bytes and flops of an average launch
+ * (a domain counts in the launches of the PCG iterations it took; finished
+ * domains are skipped by the kernels and not counted). */
 int dtc_batch_stats(dtc_batch* b, double* mv_avg_ms, int64_t* mv_launches, double* mv_bytes,
                     double* mv_flops, double* run_ms);
 int dtc_batch_destroy(dtc_batch* b);

@@ -603,19 +603,23 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
 int dtc_batch_stats(dtc_batch* b, double* mv_avg_ms, int64_t* mv_launches, double* mv_bytes, double* mv_flops,
                     double* run_ms) {
     if (!b) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL batch");
-    const double avg = b->sb.mv_count ? b->sb.mv_ms_sum / (double)b->sb.mv_count : 0.0;
-    int64_t ne = 0, nn = 0;
-    for (size_t i = 0; i < b->sb.h_elems.size(); ++i)
-        if (b->sb.h_status[i] == DTC_OK || b->sb.h_status[i] == DTC_ERR_NO_CONVERGENCE) {
-            ne += b->sb.h_elems[i];
-            nn += b->sb.h_nodes[i];
-        }
+    const int64_t n = b->sb.mv_count;
+    const double avg = n ? b->sb.mv_ms_sum / (double)n : 0.0;
+    // B_mv = 2 (3 N_n R s) + 4 N_e, F_mv = 1152 R N_e (dense-equivalent) per domain
+    // group, counted in every timed launch in which the group was still
+    // iterating (its PCG iteration count), averaged over the timed launches
     const double s = b->o.precision == DTC_FP64 ? 8.0 : 4.0;
+    double bytes = 0.0, flops = 0.0;
+    for (size_t i = 0; i < b->sb.h_elems.size() && i < b->iters.size(); ++i)
+        if (b->sb.h_status[i] == DTC_OK || b->sb.h_status[i] == DTC_ERR_NO_CONVERGENCE) {
+            const double ne = (double)b->sb.h_elems[i], nn = (double)b->sb.h_nodes[i], it = (double)b->iters[i];
+            bytes += it * (2.0 * (3.0 * nn * R * s) + 4.0 * ne);
+            flops += it * 1152.0 * R * ne;
+        }
     if (mv_avg_ms) *mv_avg_ms = avg;
-    if (mv_launches) *mv_launches = b->sb.mv_count;
-    // B_mv = 2 (3 N_n R s) + 4 N_e, F_mv = 1152 R N_e (dense-equivalent)
-    if (mv_bytes) *mv_bytes = 2.0 * (3.0 * (double)nn * R * s) + 4.0 * (double)ne;
-    if (mv_flops) *mv_flops = 1152.0 * R * (double)ne;
+    if (mv_launches) *mv_launches = n;
+    if (mv_bytes) *mv_bytes = n ? bytes / (double)n : 0.0;
+    if (mv_flops) *mv_flops = n ? flops / (double)n : 0.0;
     if (run_ms) *run_ms = b->run_ms;
     return DTC_OK;
 }

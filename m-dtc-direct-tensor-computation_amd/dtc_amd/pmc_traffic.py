@@ -3,14 +3,17 @@
     python -m dtc_amd.pmc_traffic [--batch 16] [--n 64] [--out profiles/k_matvec_traffic.json]
 
 Runs (as child processes, the program directly after `--`):
-  1. rocprofv3 --pmc FETCH_SIZE on a short bench.py run of the workload,
+  1. rocprofv3 --pmc FETCH_SIZE on one bench.py step of the workload (the
+     multigrid-preconditioned mixed solve, as timed by bench.py),
   2. rocprofv3 --pmc WRITE_SIZE on the same run,
   3. both counters on a plain 1 GiB device-to-device copy.
 FETCH_SIZE / WRITE_SIZE are reported in KiB per dispatch.  The copy reads and
 writes exactly 1 GiB, so `factor = 1 GiB / counter(copy)` converts each counter
 into bytes; traffic = FETCH * f_fetch + WRITE * f_write per dispatch of
-k_matvec<float, IN_V, OUT_MASKED_DOT>, averaged over its dispatches.  bench.py
-reports the result as roofline.traffic when its batch and domain edge match.
+k_matvec<float, IN_V, OUT_MASKED_DOT>, averaged over its dispatches (the same
+launches whose HIP-event average and algorithmic bytes bench.py reports).
+bench.py reports the result as roofline.traffic when its batch and domain edge
+match.
 """
 from __future__ import annotations
 
@@ -45,10 +48,13 @@ def _counter(dbdir: str, counter: str, match) -> tuple[float, int]:
     return sum(vals) / len(vals), len(vals)
 
 
-def _pmc(counter: str, out: str, cmd: list[str]) -> None:
-    full = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "--"] + cmd
+def _pmc(counter: str, out: str, cmd: list[str]) -> str:
+    full = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "k_matvec|copyBuffer", "-d", out, "-o", "run",
+            "--"] + cmd
     print(" ".join(full), flush=True)
-    subprocess.run(full, check=True, timeout=600, stdout=subprocess.DEVNULL, env=dict(os.environ, TMPDIR="/tmp"))
+    p = subprocess.run(full, check=True, timeout=600, stdout=subprocess.PIPE, text=True,
+                       env=dict(os.environ, TMPDIR="/tmp"))
+    return p.stdout
 
 
 def main() -> None:
@@ -58,14 +64,15 @@ def main() -> None:
     ap.add_argument("--out", default=os.path.join(ROOT, "profiles", "k_matvec_traffic.json"))
     args = ap.parse_args()
     bench = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "1", "--warmup", "0", "--batch",
-             str(args.batch), "--n", str(args.n), "--fp64-domains", "0", "--no-cpu", "--max-iter", "20",
-             "--no-check"]
+             str(args.batch), "--n", str(args.n), "--fp64-domains", "0", "--no-cpu"]
     copy = [sys.executable, "-c", COPY]
     res = {}
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             d = os.path.join(tmp, "mv_" + counter)
-            _pmc(counter, d, bench)
+            line = [l for l in _pmc(counter, d, bench).splitlines() if l.startswith("{")]
+            if line:
+                res["algorithmic"] = json.loads(line[-1])["roofline"]["algorithmic_bytes"]
             res[counter] = _counter(d, counter, lambda n: KERNEL in n.replace("(anonymous namespace)::", ""))
             d = os.path.join(tmp, "copy_" + counter)
             _pmc(counter, d, copy)
@@ -83,13 +90,14 @@ def main() -> None:
     write_kib, n_w = res["WRITE_SIZE"]
     out = {
         "kernel": "k_matvec<float, IN_V, OUT_MASKED_DOT>",
-        "workload": f"bench.py batch {args.batch}, {args.n}^3 trabecular domains (20 PCG iterations)",
+        "workload": f"bench.py batch {args.batch}, {args.n}^3 trabecular domains, one step of the mixed solve",
         "batch": args.batch, "domain_edge": args.n,
         "fetch_kib_per_dispatch": round(fetch_kib, 1), "write_kib_per_dispatch": round(write_kib, 1),
         "dispatches": [n_f, n_w],
         "copy_fetch_kib_per_gib": res["copy_FETCH_SIZE"], "copy_write_kib_per_gib": res["copy_WRITE_SIZE"],
         "fetch_factor": round(f_fetch, 4), "write_factor": round(f_write, 4),
         "traffic_bytes_per_dispatch": round(fetch_kib * 1024.0 * f_fetch + write_kib * 1024.0 * f_write),
+        "algorithmic_bytes_per_dispatch": res.get("algorithmic"),
         "command": "python -m dtc_amd.pmc_traffic --batch %d --n %d" % (args.batch, args.n),
     }
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
