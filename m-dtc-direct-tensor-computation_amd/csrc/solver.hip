@@ -224,6 +224,10 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
     __shared__ double s_wred[NCH][NW][RC];
     __shared__ V s_x[NH * 3];
     __shared__ V s_c[(OB_BIAS + 8 * NO) * 3];   // corner buffers; the bias keeps element bases non-negative
+    __shared__ short s_hn[NH];                   // halo positions that hold a node, compact
+    __shared__ unsigned char s_ol[NO];           // owned position of the brick's k-th solver node
+    __shared__ int s_wcnt[NW];
+    static_assert(NH <= MV_NT, "one thread per halo position");
 
     const BrickInfo B = A.bricks[blockIdx.x];
     if (A.done && A.done[B.dom]) return;
@@ -233,17 +237,59 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
     V* yv = reinterpret_cast<V*>(yout);
     const int c0 = (int)blockIdx.y * NCH / (int)gridDim.y, c1 = ((int)blockIdx.y + 1) * NCH / (int)gridDim.y;
 
-    // chunk-invariant 16-byte offsets of the halo segments this thread stages
-    int pf_off[NLD];
+    // Only halo positions and owned positions that hold a node carry work: on
+    // sparse (trabecular) domains a brick holds about a third of its 128
+    // positions.  Compact lists: s_hn = halo positions with a node (wave ballots
+    // and a prefix over the waves), s_ol = owned position of solver node
+    // node_begin + k (owned nodes are numbered in position order).
+    const int n_own = B.node_end - B.node_begin;
+    int nh = 0;
+    {
+        const int q = tid < NH ? halo[tid] : -1;
+        if (tid < NH) s_node[tid] = q;
+        const bool ex = q >= 0;
+        const unsigned long long bal = __ballot(ex);
+        const int lane = tid & 63, wv = tid >> 6;
+        if (lane == 0) s_wcnt[wv] = __popcll(bal);
+        const int hx = tid % HX, hy = (tid / HX) % HY, hz = tid / (HX * HY);
+        if (ex && hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ)
+            s_ol[(q & NODE_MASK) - B.node_begin] = (unsigned char)((hx - 1) + BX * ((hy - 1) + BY * (hz - 1)));
+        if (tid < NO) s_nm[tid] = A.nmask[(int64_t)blockIdx.x * NO + tid];
+        __syncthreads();
+        int base = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const int c = s_wcnt[w];
+            if (w < wv) base += c;
+            nh += c;
+        }
+        if (ex) s_hn[base + __popcll(bal & ((1ull << lane) - 1))] = (short)tid;
+    }
+    const int ne = B.e_cnt;
+    {
+        const T* __restrict__ eE = static_cast<const T*>(A.eE) + B.e_off;
+        const int* __restrict__ ei = A.einfo + B.e_off;
+        for (int i = tid; i < ne; i += MV_NT) {
+            s_info[i] = ei[i];
+            s_E[i] = eE[i];
+        }
+    }
+    __syncthreads();
+
+    // staged (halo node, direction) items of this thread: i = tid + k * MV_NT < 3 nh,
+    // 16-byte offset in x and destination in s_x
+    int pf_off[NLD], pf_xi[NLD];
 #pragma unroll
     for (int k = 0; k < NLD; ++k) {
         const int i = tid + k * MV_NT;
-        int o = -1;
-        if (IN == IN_V && i < NH * 3) {
-            const int q = halo[i / 3];
-            if (q >= 0) o = ((q & NODE_MASK) * 3 + i % 3) * NCH;
+        int o = -1, xi = 0;
+        if (i < 3 * nh) {
+            const int h = s_hn[i / 3], d = i % 3;
+            o = ((s_node[h] & NODE_MASK) * 3 + d) * NCH;
+            xi = h * 3 + d;
         }
         pf_off[k] = o;
+        pf_xi[k] = xi;
     }
     // x segments of the next chunk to stage, loaded one chunk ahead
     V reg[NLD];
@@ -257,32 +303,18 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
         }
     };
     if (IN == IN_V) prefetch(c0);
-    for (int i = tid; i < NH; i += MV_NT) s_node[i] = halo[i];
-    if (tid < NO) s_nm[tid] = A.nmask[(int64_t)blockIdx.x * NO + tid];
-    const int ne = B.e_cnt;
-    {
-        const T* __restrict__ eE = static_cast<const T*>(A.eE) + B.e_off;
-        const int* __restrict__ ei = A.einfo + B.e_off;
-        for (int i = tid; i < ne; i += MV_NT) {
-            s_info[i] = ei[i];
-            s_E[i] = eE[i];
-        }
-    }
-    __syncthreads();
 
     // s_x <- x of `chunk` (IN_V: from the prefetch registers; IN_U: x + BC values)
     auto stage = [&](int chunk) {
         if (IN == IN_V) {
 #pragma unroll
-            for (int k = 0; k < NLD; ++k) {
-                const int i = tid + k * MV_NT;
-                if (i < NH * 3) s_x[i] = reg[k];
-            }
+            for (int k = 0; k < NLD; ++k)
+                if (pf_off[k] >= 0) s_x[pf_xi[k]] = reg[k];
         } else {
             const DomInfo D = A.doms[B.dom];
             T* sx = reinterpret_cast<T*>(s_x);
-            for (int i = tid; i < NH * SX; i += MV_NT) {
-                const int h = i / SX, rem = i % SX, d = rem / RC, r = rem % RC;
+            for (int i = tid; i < nh * SX; i += MV_NT) {
+                const int h = s_hn[i / SX], rem = i % SX, d = rem / RC, r = rem % RC;
                 const int lc = chunk * RC + r;
                 const int q = s_node[h];
                 T v = T(0);
@@ -297,12 +329,13 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                         v = xin[((int64_t)(q & NODE_MASK) * 3 + d) * R + lc];
                     }
                 }
-                sx[i] = v;
+                sx[h * SX + rem] = v;
             }
         }
     };
 
-    // chunk-invariant gather items of this thread: owned (node, direction) i = tid + j * MV_NT
+    // chunk-invariant gather items of this thread: (owned solver node, direction)
+    // i = tid + j * MV_NT < 3 n_own
     constexpr int NG = (NO * 3 + MV_NT - 1) / MV_NT;
     int g_off[NG], g_xi[NG];
     unsigned g_m[NG];
@@ -312,14 +345,12 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
         const int i = tid + j * MV_NT;
         int off = -1, xi = 0;
         unsigned m = 0;
-        if (i < NO * 3) {
-            const int o = i / 3, d = i % 3, h = ohalo(o);
+        if (i < 3 * n_own) {
+            const int o = s_ol[i / 3], d = i % 3, h = ohalo(o);
             const int q = s_node[h];
-            if (q >= 0) {
-                off = ((q & NODE_MASK) * 3 + d) * NCH;
-                xi = h * 3 + d;
-                m = (unsigned)s_nm[o] | ((q & BC_BIT) ? 256u : 0u) | ((unsigned)o << 9) | ((unsigned)d << 16);
-            }
+            off = ((q & NODE_MASK) * 3 + d) * NCH;
+            xi = h * 3 + d;
+            m = (unsigned)s_nm[o] | ((q & BC_BIT) ? 256u : 0u) | ((unsigned)o << 9) | ((unsigned)d << 16);
         }
         g_off[j] = off;
         g_xi[j] = xi;
@@ -463,8 +494,11 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
         }
         if (OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) {
             // every lane holds the RC load cases of the chunk: fixed-order wave sums
+            // (waves past the last gather item hold zeros)
+            if ((tid & ~63) < 3 * n_own) {
 #pragma unroll
-            for (int w = 0; w < RC; ++w) acc[w] = wave_sum64(acc[w]);
+                for (int w = 0; w < RC; ++w) acc[w] = wave_sum64(acc[w]);
+            }
             if (tid % 64 == 0)
 #pragma unroll
                 for (int w = 0; w < RC; ++w) s_wred[chunk][tid / 64][w] = acc[w];
