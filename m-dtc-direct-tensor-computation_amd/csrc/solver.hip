@@ -122,11 +122,8 @@ __device__ inline void walsh_apply(const WK<T>& k, const T w[8][3], T y[8][3]) {
 }
 
 // ---------------------------------------------------------------- wave sums
-// FP64 sum over the 64 lanes of a wave with DPP lane moves (VALU, no LDS round
-// trip): lanes xor 1 and xor 2 inside quads, then the half-row and row mirrors
-// leave every lane of a 16-lane row with the row sum (each step adds a value
-// and its mirror image, so all lanes hold the same bits); the four row sums are
-// added in row order.  Fixed order: deterministic.
+// DPP lane move of an FP64 value (two 32-bit moves); lanes whose source lies
+// outside their 16-lane row receive 0.
 template <int CTRL>
 __device__ inline double dpp_f64(double v) {
     const long long b = __double_as_longlong(v);
@@ -134,18 +131,40 @@ __device__ inline double dpp_f64(double v) {
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
-__device__ inline double readlane_f64(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ inline double wave_sum64(double v) {
-    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
-    v += dpp_f64<0x141>(v);   // row_half_mirror
-    v += dpp_f64<0x140>(v);   // row_mirror
-    return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
+// RC partial sums per lane (RC = 4 or 2) -> per 16-lane row, the row sum of each
+// of the RC values, in fixed order: log2(RC) halving exchanges between lane pairs
+// (each lane keeps half of the values and adds its partner's copies), then
+// row_shr by RC, 2 RC, .. 8 accumulates each residue class of the row into its
+// last lanes.  Returns true on the RC lanes 16-RC..15 of each row, which then
+// hold the sum of value `*lc` in `*v`.
+template <int RC>
+__device__ inline bool row_sums(const double (&a)[RC], int lane, double* v, int* lc) {
+    const bool b0 = lane & 1, b1 = lane & 2;
+    double k;
+    int l;
+    if constexpr (RC == 4) {
+        double k0 = b0 ? a[2] : a[0], k1 = b0 ? a[3] : a[1];
+        const double s0 = b0 ? a[0] : a[2], s1 = b0 ? a[1] : a[3];
+        k0 += dpp_f64<0xB1>(s0);   // quad_perm [1,0,3,2]
+        k1 += dpp_f64<0xB1>(s1);
+        k = b1 ? k1 : k0;
+        const double s = b1 ? k0 : k1;
+        k += dpp_f64<0x4E>(s);     // quad_perm [2,3,0,1]
+        l = 2 * (int)b0 + (int)b1;
+        k += dpp_f64<0x114>(k);    // row_shr:4
+        k += dpp_f64<0x118>(k);    // row_shr:8
+    } else {
+        static_assert(RC == 2, "two or four load cases per chunk");
+        k = b0 ? a[1] : a[0];
+        k += dpp_f64<0xB1>(b0 ? a[0] : a[1]);
+        l = (int)b0;
+        k += dpp_f64<0x112>(k);    // row_shr:2
+        k += dpp_f64<0x114>(k);
+        k += dpp_f64<0x118>(k);
+    }
+    *v = k;
+    *lc = l;
+    return (lane & 15) >= 16 - RC;
 }
 
 // ---------------------------------------------------------------- matvec
@@ -221,7 +240,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
     __shared__ int s_info[NET];
     __shared__ T s_E[NET];
     __shared__ unsigned char s_nm[NO];
-    __shared__ double s_wred[NCH][NW][RC];
+    __shared__ double s_wred[NCH][NW * 4][RC];   // per chunk, wave row and load case
     __shared__ V s_x[NH * 3];
     __shared__ V s_c[(OB_BIAS + 8 * NO) * 3];   // corner buffers; the bias keeps element bases non-negative
     __shared__ short s_hn[NH];                   // halo positions that hold a node, compact
@@ -359,6 +378,12 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
         if (OUT == OUT_CHEB && off >= 0 && !(m & 256u)) e_di[j] = reinterpret_cast<const T*>(A.dinv)[off / NCH];
     }
 
+    // corner slots that no element writes (element absent) are zero for the whole
+    // launch, so the gather adds all 8 corner buffers without tests
+    for (int i = tid; i < n_own * 24; i += MV_NT) {
+        const int o = s_ol[i / 24], b = (i / 3) % 8, d = i % 3;
+        if (!((s_nm[o] >> b) & 1)) vzero(s_c[(OB_BIAS + b * NO + o) * 3 + d]);
+    }
     stage(c0);
     if (IN == IN_V && c0 + 1 < c1) prefetch(c0 + 1);
     __syncthreads();
@@ -443,8 +468,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             V y;
             vzero(y);
 #pragma unroll
-            for (int b = 0; b < 8; ++b)
-                if (m & (1u << b)) vadd(y, s_c[(OB_BIAS + b * NO + o) * 3 + d]);
+            for (int b = 0; b < 8; ++b) vadd(y, s_c[(OB_BIAS + b * NO + o) * 3 + d]);
             const bool bc = (m & 256u) != 0;
             const int vi = g_off[j] + chunk;
             if (OUT == OUT_MASKED_DOT) {
@@ -493,15 +517,14 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             yv[vi] = y;
         }
         if (OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) {
-            // every lane holds the RC load cases of the chunk: fixed-order wave sums
+            // every lane holds the RC load cases of the chunk: fixed-order row sums
             // (waves past the last gather item hold zeros)
-            if ((tid & ~63) < 3 * n_own) {
-#pragma unroll
-                for (int w = 0; w < RC; ++w) acc[w] = wave_sum64(acc[w]);
-            }
-            if (tid % 64 == 0)
-#pragma unroll
-                for (int w = 0; w < RC; ++w) s_wred[chunk][tid / 64][w] = acc[w];
+            const int lane = tid & 63, row = (tid >> 6) * 4 + (lane >> 4);
+            double v = 0.0;
+            int lc = lane & (RC - 1);
+            bool out = (lane & 15) >= 16 - RC;
+            if ((tid & ~63) < 3 * n_own) out = row_sums<RC>(acc, lane, &v, &lc);
+            if (out) s_wred[chunk][row][lc] = v;
         }
         __syncthreads();
     }
@@ -510,7 +533,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
         const int ch = tid / RC, r = tid % RC;
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) s += s_wred[ch][w][r];
+        for (int w = 0; w < NW * 4; ++w) s += s_wred[ch][w][r];
         A.partial[(int64_t)blockIdx.x * R + tid] = s;
     }
 }
