@@ -206,9 +206,15 @@ struct MvArgs {
     // OUT_CHEB / OUT_BRES only
     const void* bvec;           // right-hand side b
     const void* dinv;           // Jacobi scaling
-    void* dvec;                 // Chebyshev direction (read and written)
+    void* dvec;                 // Chebyshev direction
     float c1, c2;               // Chebyshev coefficients of this step
+    int cflags;                 // CH_* : where the previous direction comes from, whether d is stored
 };
+// OUT_CHEB direction handling: the previous direction d is read from dvec
+// (CH_READ_D), or equals the current iterate (CH_D_FROM_X: the step after a
+// zero-start step, where x_1 = d_0), or is absent (c1 = 0); the new direction
+// is stored only when a later step reads it (CH_WRITE_D).
+constexpr int CH_READ_D = 1, CH_D_FROM_X = 2, CH_WRITE_D = 4;
 
 // y = K x for every brick of the batch.  One workgroup per (brick, chunk
 // range): blockIdx.y splits the NCH chunks into gridDim.y equal ranges, so that
@@ -405,7 +411,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             if (g_off[j] >= 0) {
                 const int vi = g_off[j] + chunk;
                 if (OUT == OUT_BRES || OUT == OUT_CHEB) e_b[j] = reinterpret_cast<const V*>(A.bvec)[vi];
-                if (OUT == OUT_CHEB) e_d[j] = reinterpret_cast<const V*>(A.dvec)[vi];
+                if (OUT == OUT_CHEB && (A.cflags & CH_READ_D)) e_d[j] = reinterpret_cast<const V*>(A.dvec)[vi];
                 if (OUT == OUT_MASKED_DOT || OUT == OUT_CHEB) e_x[j] = s_x[g_xi[j]];
             }
         }
@@ -497,7 +503,8 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                     vzero(xn);
                     if (!bc) {
                         const T di = e_di[j];
-                        const V dv = e_d[j], xv = e_x[j];
+                        const V xv = e_x[j];
+                        const V dv = (A.cflags & CH_D_FROM_X) ? xv : e_d[j];
                         T* dnp = reinterpret_cast<T*>(&dn);
                         T* xnp = reinterpret_cast<T*>(&xn);
 #pragma unroll
@@ -507,7 +514,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                             xnp[w] = vcomp(xv, w) + dnp[w];
                         }
                     }
-                    *dp = dn;
+                    if (A.cflags & CH_WRITE_D) *dp = dn;
                     y = xn;
                     if (A.partial)
 #pragma unroll
@@ -1320,6 +1327,7 @@ __global__ __launch_bounds__(256) void k_prolong(const BrickInfo* __restrict__ f
 // one Chebyshev-Jacobi step on the owned nodes of each brick:
 //   zero:      d = c2 D^-1 b,              x = d        (first step from x = 0; t unused)
 //   otherwise: d = c1 d + c2 D^-1 (b - t), x = x + d    (t = A x)
+// d == null: the new direction is not stored (zero start: it equals x).
 // prz (optional): per-brick sums of b . x per load case
 __global__ __launch_bounds__(NT_UPD) void k_cheb(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
                                                  const float* __restrict__ b, const float* __restrict__ t,
@@ -1344,7 +1352,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cheb(const BrickInfo* __restrict__ b
             dv = c1 * d[idx] + c2 * di * (bv - t[idx]);
             xv = x[idx] + dv;
         }
-        d[idx] = dv;
+        if (d) d[idx] = dv;
         x[idx] = xv;
         a1 += (double)bv * (double)xv;
     }
@@ -1888,9 +1896,9 @@ struct MvTopo {
 template <int OUT>
 int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float* x, float* y, const int* done,
                 const float* bv = nullptr, const float* dinv = nullptr, float* dv = nullptr, float c1 = 0.f,
-                float c2 = 0.f, double* partial = nullptr) {
+                float c2 = 0.f, double* partial = nullptr, int cflags = CH_READ_D | CH_WRITE_D) {
     if (T.n_bricks == 0) return DTC_OK;
-    MvArgs a{T.bricks, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2};
+    MvArgs a{T.bricks, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2, cflags};
     const dim3 grid(T.n_bricks, chunk_split(T.n_bricks, R / MvCfg<float>::RC));
     k_matvec<float, IN_V, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
     DTC_HIP(hipGetLastError());
@@ -1977,12 +1985,14 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
             }
             double* dk = k == m - 1 ? dot : nullptr;
             if (zero && k == 0) {
-                k_cheb<<<v.T.n_bricks, NT_UPD, 0, s>>>(v.T.bricks, done, v.b, nullptr, v.dinv, v.d, v.buf[cur[l]], c1,
-                                                       c2, 1, dk);
+                // x_1 = d_0: the next step takes its previous direction from x
+                k_cheb<<<v.T.n_bricks, NT_UPD, 0, s>>>(v.T.bricks, done, v.b, nullptr, v.dinv, nullptr, v.buf[cur[l]],
+                                                       c1, c2, 1, dk);
                 DTC_HIP(hipGetLastError());
             } else {
+                const int fl = (k == 0 ? 0 : (zero && k == 1) ? CH_D_FROM_X : CH_READ_D) | (k < m - 1 ? CH_WRITE_D : 0);
                 DTC_TRY(launch_mv32<OUT_CHEB>(v.T, s, wk, v.buf[cur[l]], v.buf[1 - cur[l]], done, v.b, v.dinv, v.d,
-                                              c1, c2, dk));
+                                              c1, c2, dk, fl));
                 cur[l] = 1 - cur[l];
             }
         }
