@@ -1239,27 +1239,43 @@ __device__ inline void f4_axpy(float4& s, float w, const float4& a) {
     s.x += w * a.x; s.y += w * a.y; s.z += w * a.z; s.w += w * a.w;
 }
 
+// s_pos[k] = position (0..NO-1) of the brick's k-th solver node (owned nodes are
+// numbered in position order), so the transfer loops visit occupied positions only
+__device__ inline int brick_positions(const BrickInfo& B, const DomInfo& D, const int* __restrict__ l2n,
+                                      unsigned char* s_pos) {
+    const int t = threadIdx.x;
+    if (t < NO) {
+        const int gx = B.x0 + t % BX, gy = B.y0 + (t / BX) % BY, gz = B.z0 + t / (BX * BY);
+        const int Lx = D.nx + 1, Ly = D.ny + 1;
+        if (gx <= D.nx && gy <= D.ny && gz <= D.nz) {
+            const int q = l2n[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
+            if (q >= 0) s_pos[q - B.node_begin] = (unsigned char)t;
+        }
+    }
+    __syncthreads();
+    return B.node_end - B.node_begin;
+}
+
 // b_c = P^T r_f: trilinear weights over the free (interior) fine nodes; one
 // workgroup per (coarse brick, blockIdx.y-th share of its nodes)
 __global__ __launch_bounds__(256) void k_restrict(const BrickInfo* __restrict__ cbr, const DomInfo* __restrict__ cdom,
                                                   const int* __restrict__ cl2n, const DomInfo* __restrict__ fdom,
                                                   const int* __restrict__ fl2n, const float* __restrict__ rf,
                                                   const int* __restrict__ done, float* __restrict__ bc) {
+    __shared__ unsigned char s_pos[NO];
     const BrickInfo B = cbr[blockIdx.x];
     if (done && done[B.dom]) return;
     const int tid = threadIdx.x;
-    if (tid >= T_NPP * T_LPN) return;
     const DomInfo C = cdom[B.dom], F = fdom[B.dom];
+    const int n = brick_positions(B, C, cl2n, s_pos);
+    if (tid >= T_NPP * T_LPN) return;
     const int slot = tid / T_LPN, j = tid % T_LPN;
-    const int cLx = C.nx + 1, cLy = C.ny + 1;
     const int fLx = F.nx + 1, fLy = F.ny + 1;
     const float4* rf4 = reinterpret_cast<const float4*>(rf);
     float4* bc4 = reinterpret_cast<float4*>(bc);
-    for (int o = slot + (int)blockIdx.y * T_NPP; o < NO; o += T_NPP * (int)gridDim.y) {
+    for (int k = slot + (int)blockIdx.y * T_NPP; k < n; k += T_NPP * (int)gridDim.y) {
+        const int o = s_pos[k], qc = B.node_begin + k;
         const int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
-        if (gx > C.nx || gy > C.ny || gz > C.nz) continue;
-        const int qc = cl2n[C.lat_off + gx + (int64_t)cLx * (gy + (int64_t)cLy * gz)];
-        if (qc < 0) continue;
         float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
         if (!is_bc(C, gx, gy, gz)) {
 #pragma unroll
@@ -1289,21 +1305,21 @@ __global__ __launch_bounds__(256) void k_prolong(const BrickInfo* __restrict__ f
                                                  const int* __restrict__ fl2n, const DomInfo* __restrict__ cdom,
                                                  const int* __restrict__ cl2n, const float* __restrict__ xc,
                                                  const int* __restrict__ done, float* __restrict__ xf) {
+    __shared__ unsigned char s_pos[NO];
     const BrickInfo B = fbr[blockIdx.x];
     if (done && done[B.dom]) return;
     const int tid = threadIdx.x;
-    if (tid >= T_NPP * T_LPN) return;
     const DomInfo F = fdom[B.dom], C = cdom[B.dom];
+    const int n = brick_positions(B, F, fl2n, s_pos);
+    if (tid >= T_NPP * T_LPN) return;
     const int slot = tid / T_LPN, j = tid % T_LPN;
-    const int fLx = F.nx + 1, fLy = F.ny + 1;
     const int cLx = C.nx + 1, cLy = C.ny + 1;
     const float4* xc4 = reinterpret_cast<const float4*>(xc);
     float4* xf4 = reinterpret_cast<float4*>(xf);
-    for (int o = slot + (int)blockIdx.y * T_NPP; o < NO; o += T_NPP * (int)gridDim.y) {
+    for (int k = slot + (int)blockIdx.y * T_NPP; k < n; k += T_NPP * (int)gridDim.y) {
+        const int o = s_pos[k], qf = B.node_begin + k;
         const int gx = B.x0 + o % BX, gy = B.y0 + (o / BX) % BY, gz = B.z0 + o / (BX * BY);
         if (!(gx > 0 && gy > 0 && gz > 0 && gx < F.nx && gy < F.ny && gz < F.nz)) continue;   // free nodes only
-        const int qf = fl2n[F.lat_off + gx + (int64_t)fLx * (gy + (int64_t)fLy * gz)];
-        if (qf < 0) continue;
         const int X0 = gx >> 1, Y0 = gy >> 1, Z0 = gz >> 1;
         const int ox = gx & 1, oy = gy & 1, oz = gz & 1;
         const float w = (ox ? 0.5f : 1.f) * (oy ? 0.5f : 1.f) * (oz ? 0.5f : 1.f);
