@@ -129,6 +129,17 @@ def main():
     log("PCG iterations per domain: " + " ".join(str(r["iterations"]) for r in res) +
         "; FP64 residual replacements: " + " ".join(str(r["outer_iterations"]) for r in res))
 
+    # ---- sensitivity dC/dE of every domain (reuses the stored load-case solutions)
+    G = np.eye(6) + 0.1 * np.ones((6, 6))
+    _sync()
+    g0 = time.perf_counter()
+    for i, r in enumerate(res):
+        if r["status"] == 0:
+            b.grad(i, G)
+    _sync()
+    t_grad = time.perf_counter() - g0
+    log(f"gradient of all {len(doms)} domains: {t_grad * 1e3:.1f} ms ({t_grad / (dt / args.steps):.3f} x one step)")
+
     # ---- plain FP64 GPU form, one domain at a time
     t64, C64, it64 = 0.0, [], []
     for i in range(min(args.fp64_domains, len(doms))):
@@ -178,6 +189,7 @@ def main():
         "fp64_plain_domains_per_s": round(fp64_rate, 3), "speedup_vs_fp64": round(value / fp64_rate, 2),
         "tensor_rel_err_vs_fp64": float(f"{terr:.3e}"), "pcg_iters_mixed": float(mixed_iters),
         "pcg_iters_fp64": float(np.mean(it64)) if it64 else None,
+        "grad_over_forward": round(t_grad / (dt / args.steps), 4),
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

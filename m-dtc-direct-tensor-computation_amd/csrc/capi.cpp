@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -257,15 +258,29 @@ struct dtc_batch {
     Stream st;
     std::vector<std::unique_ptr<DevMem>> phi, mod;
     std::vector<double> escale;
-    std::vector<int> g0, ng;       // first solver group and group count of each domain (24 load cases per group)
-    SolverBatch sb;
+    // Domains are solved in sub-batches of consecutive domains whose solver
+    // nodes stay below NODE_CAP (32-bit vector indexing); each sub-batch keeps
+    // its topology and solutions for the sensitivity pass.
+    struct Sub {
+        SolverBatch sb;
+        std::vector<double> ff;
+        std::vector<int64_t> iters, outer;
+        std::vector<double> relres;
+    };
+    std::vector<std::unique_ptr<Sub>> subs;
+    std::vector<int> sbi, g0, ng;  // sub-batch, first solver group and group count of each domain
+    double nu = 0.3, a0 = 1.0;
+    std::vector<double> khat20;    // HEX20 micro: E = 1 element stiffness
     MeshDev mesh;
-    std::vector<double> ff, EE;
-    std::vector<int64_t> iters, outer;
-    std::vector<double> relres;
+    std::vector<double> EE;
     double run_ms = 0.0;
     bool ran = false;
 };
+
+namespace {
+// solver nodes of one sub-batch: below 2^31 / 72 (FP32 vectors [node][3][24])
+constexpr int64_t NODE_CAP = 24000000;
+}
 
 extern "C" {
 
@@ -449,13 +464,11 @@ int dtc_batch_create(const dtc_domain_desc* d, int64_t n_dom, const dtc_solve_op
         b->d[i].modulus = nullptr;
     }
     DTC_HIP(hipStreamSynchronize(b->st.s));
-    b->sb.stream = b->st.s;
-    b->sb.nu = d[0].nu;
-    b->sb.a = a0;
-    b->sb.micro20 = d[0].micro_nen == 20;
-    if (b->sb.micro20) {
-        b->sb.khat20.assign(3600, 0.0);
-        DTC_TRY(element_stiffness20(1.0, d[0].nu, a0, b->sb.khat20.data()));
+    b->nu = d[0].nu;
+    b->a0 = a0;
+    if (d[0].micro_nen == 20) {
+        b->khat20.assign(3600, 0.0);
+        DTC_TRY(element_stiffness20(1.0, d[0].nu, a0, b->khat20.data()));
     }
     *out = b.release();
     return DTC_OK;
@@ -467,40 +480,71 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
     hipStream_t s = b->st.s;
     DTC_HIP(hipStreamSynchronize(s));
     const auto t0 = std::chrono::steady_clock::now();
-    b->sb.timing = true;
     // each domain is solved as groups of 24 macro load cases (1 group for the
-    // HEX08 macro element, 3 for the 60 load cases of the HEX20 one)
-    std::vector<dtc_domain_desc> gd;
+    // HEX08 macro element, 3 for the 60 load cases of the HEX20 one); domains are
+    // dealt to sub-batches by an upper bound of their solver nodes (lattice size)
+    b->sbi.assign(D, 0);
     b->g0.assign(D, 0);
     b->ng.assign(D, 0);
-    for (int i = 0; i < D; ++i) {
-        b->g0[i] = (int)gd.size();
-        b->ng[i] = (macro_lcs(b->d[i].macro_order) + R - 1) / R;
-        for (int g = 0; g < b->ng[i]; ++g) gd.push_back(b->d[i]);
-    }
-    DTC_TRY(batch_begin(b->sb, gd.data(), (int64_t)gd.size()));
-    for (int i = 0; i < D; ++i) {
-        const dtc_domain_desc& d = b->d[i];
-        int rc = mesh_build_device(b->phi[i]->p, d.phi_kind, d.n, d.llimit, d.micro_nen, b->mesh, s);
-        if (rc == DTC_ERR_EMPTY_MESH) {
-            for (int g = 0; g < b->ng[i]; ++g) DTC_TRY(batch_add_empty(b->sb, d, DTC_ERR_EMPTY_MESH, R * g));
-            continue;
+    std::vector<int> first(1, 0);
+    {
+        int64_t cap = NODE_CAP;   // DTC_SUBBATCH_NODES lowers it (smaller sub-batches)
+        if (const char* e = std::getenv("DTC_SUBBATCH_NODES")) cap = std::max<int64_t>(1, std::min<int64_t>(cap, std::atoll(e)));
+        int64_t acc = 0;
+        for (int i = 0; i < D; ++i) {
+            const dtc_domain_desc& d = b->d[i];
+            b->ng[i] = (macro_lcs(d.macro_order) + R - 1) / R;
+            const int64_t m = d.micro_nen == 20 ? 2 : 1;
+            const int64_t est = (m * d.n[0] + 1) * (m * d.n[1] + 1) * (m * d.n[2] + 1) * b->ng[i];
+            if (acc > 0 && acc + est > cap) {
+                first.push_back(i);
+                acc = 0;
+            }
+            acc += est;
         }
-        DTC_TRY(rc);
-        const double* mod = b->mod[i]->p ? b->mod[i]->as<double>() : nullptr;
-        for (int g = 0; g < b->ng[i]; ++g) {
-            if (b->sb.micro20) DTC_TRY(batch_add_domain20(b->sb, b->mesh, d, mod, b->escale[i], R * g, s));
-            else DTC_TRY(batch_add_domain(b->sb, b->mesh, d, mod, b->escale[i], R * g, s));
-        }
+        first.push_back(D);
     }
-    DTC_TRY(batch_finalize(b->sb, b->o.precision));
-    // the multigrid hierarchy coarsens the voxel lattice: HEX20 micro meshes use Jacobi
-    b->sb.precond = (b->o.precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG && !b->sb.micro20) ? DTC_PC_MG
-                                                                                                      : DTC_PC_JACOBI;
-    b->sb.mg_degree = b->o.mg_degree > 0 ? b->o.mg_degree : 2;
-    if (b->sb.precond == DTC_PC_MG) DTC_TRY(batch_build_mg(b->sb, 8));
-    DTC_TRY(batch_solve(b->sb, b->o, b->iters, b->outer, b->relres));
-    DTC_TRY(batch_forces(b->sb, b->ff));
+    const int NS = (int)first.size() - 1;
+    while ((int)b->subs.size() < NS) b->subs.emplace_back(new dtc_batch::Sub);
+    for (int k = 0; k < NS; ++k) {
+        dtc_batch::Sub& S = *b->subs[k];
+        SolverBatch& sb = S.sb;
+        sb.stream = s;
+        sb.nu = b->nu;
+        sb.a = b->a0;
+        sb.micro20 = b->d[0].micro_nen == 20;
+        sb.khat20 = b->khat20;
+        sb.timing = true;
+        std::vector<dtc_domain_desc> gd;
+        for (int i = first[k]; i < first[k + 1]; ++i) {
+            b->sbi[i] = k;
+            b->g0[i] = (int)gd.size();
+            for (int g = 0; g < b->ng[i]; ++g) gd.push_back(b->d[i]);
+        }
+        DTC_TRY(batch_begin(sb, gd.data(), (int64_t)gd.size()));
+        for (int i = first[k]; i < first[k + 1]; ++i) {
+            const dtc_domain_desc& d = b->d[i];
+            int rc = mesh_build_device(b->phi[i]->p, d.phi_kind, d.n, d.llimit, d.micro_nen, b->mesh, s);
+            if (rc == DTC_ERR_EMPTY_MESH) {
+                for (int g = 0; g < b->ng[i]; ++g) DTC_TRY(batch_add_empty(sb, d, DTC_ERR_EMPTY_MESH, R * g));
+                continue;
+            }
+            DTC_TRY(rc);
+            const double* mod = b->mod[i]->p ? b->mod[i]->as<double>() : nullptr;
+            for (int g = 0; g < b->ng[i]; ++g) {
+                if (sb.micro20) DTC_TRY(batch_add_domain20(sb, b->mesh, d, mod, b->escale[i], R * g, s));
+                else DTC_TRY(batch_add_domain(sb, b->mesh, d, mod, b->escale[i], R * g, s));
+            }
+        }
+        DTC_TRY(batch_finalize(sb, b->o.precision));
+        // the multigrid hierarchy coarsens the voxel lattice: HEX20 micro meshes use Jacobi
+        sb.precond = (b->o.precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG && !sb.micro20) ? DTC_PC_MG
+                                                                                                     : DTC_PC_JACOBI;
+        sb.mg_degree = b->o.mg_degree > 0 ? b->o.mg_degree : 2;
+        if (sb.precond == DTC_PC_MG) DTC_TRY(batch_build_mg(sb, 8));
+        DTC_TRY(batch_solve(sb, b->o, S.iters, S.outer, S.relres));
+        DTC_TRY(batch_forces(sb, S.ff));
+    }
     b->EE.assign((size_t)D * 36, 0.0);
     std::vector<dtc_domain_result> out(D);
     std::vector<int> solved;
@@ -509,19 +553,20 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         dtc_domain_result& r = out[i];
         std::memset(&r, 0, sizeof(r));
         const dtc_domain_desc& d = b->d[i];
+        const dtc_batch::Sub& S = *b->subs[b->sbi[i]];
         const int gi = b->g0[i], RM = macro_lcs(d.macro_order), FD = 3 * macro_nodes(d.macro_order);
-        r.status = b->sb.h_status[gi];
+        r.status = S.sb.h_status[gi];
         for (int g = 0; g < b->ng[i]; ++g) {
-            const int st = b->sb.h_status[gi + g];
+            const int st = S.sb.h_status[gi + g];
             if (r.status == DTC_OK && st != DTC_OK) r.status = st;
-            r.iterations = std::max<int64_t>(r.iterations, b->iters[gi + g]);
-            r.outer_iterations = std::max<int64_t>(r.outer_iterations, b->outer[gi + g]);
-            const double rr = b->relres[gi + g];
+            r.iterations = std::max<int64_t>(r.iterations, S.iters[gi + g]);
+            r.outer_iterations = std::max<int64_t>(r.outer_iterations, S.outer[gi + g]);
+            const double rr = S.relres[gi + g];
             r.relres_max = (std::isnan(rr) || std::isnan(r.relres_max)) ? NAN : std::max(r.relres_max, rr);
         }
-        r.n_elems = b->sb.h_elems[gi];
-        r.n_nodes = b->sb.h_nodes[gi];
-        r.n_bnodes = b->sb.h_bnodes[gi];
+        r.n_elems = S.sb.h_elems[gi];
+        r.n_nodes = S.sb.h_nodes[gi];
+        r.n_bnodes = S.sb.h_bnodes[gi];
         double xdp[3], v = 1.0;
         for (int k = 0; k < 3; ++k) {
             xdp[k] = (double)d.n[k] * d.delta[k];
@@ -533,7 +578,7 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         for (int g = 0; g < b->ng[i]; ++g)
             for (int l = 0; l < R && R * g + l < RM; ++l)
                 for (int k = 0; k < FD; ++k)
-                    r.ff[k + RM * (R * g + l)] = b->ff[((size_t)(gi + g) * R + l) * FFS + k];
+                    r.ff[k + RM * (R * g + l)] = S.ff[((size_t)(gi + g) * R + l) * FFS + k];
         DTC_TRY(effective(r.ff, d.macro_order, xdp, d.rve_strain, r.K_macro, r.C, r.EE));
         std::memcpy(&b->EE[(size_t)i * 36], r.EE, sizeof(r.EE));
         solved.push_back(i);
@@ -571,8 +616,9 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
     if (d.macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements the HEX08 macro element");
     if (d.micro_nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements HEX08 micro elements");
     const int gi = b->g0[i];
+    SolverBatch& sb = b->subs[b->sbi[i]]->sb;
     const size_t nv = (size_t)d.n[0] * d.n[1] * d.n[2];
-    if (b->sb.h_status[gi] != DTC_OK && b->sb.h_status[gi] != DTC_ERR_NO_CONVERGENCE) {
+    if (sb.h_status[gi] != DTC_OK && sb.h_status[gi] != DTC_ERR_NO_CONVERGENCE) {
         std::memset(dL_dE, 0, nv * 8);
         return DTC_OK;
     }
@@ -594,7 +640,7 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
             M[bb * 6 + j] = s * MEPS[j];
         }
     std::vector<double> g(nv);
-    DTC_TRY(batch_grad(b->sb, gi, M, g.data()));
+    DTC_TRY(batch_grad(sb, gi, M, g.data()));
     const double sc = b->escale[i];   // d(emod)/d(E_v)
     for (size_t v = 0; v < nv; ++v) dL_dE[v] = g[v] * sc;
     return DTC_OK;
@@ -603,19 +649,24 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
 int dtc_batch_stats(dtc_batch* b, double* mv_avg_ms, int64_t* mv_launches, double* mv_bytes, double* mv_flops,
                     double* run_ms) {
     if (!b) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL batch");
-    const int64_t n = b->sb.mv_count;
-    const double avg = n ? b->sb.mv_ms_sum / (double)n : 0.0;
+    int64_t n = 0;
+    double ms = 0.0, bytes = 0.0, flops = 0.0;
     // B_mv = 2 (3 N_n R s) + 4 N_e, F_mv = 1152 R N_e (dense-equivalent) per domain
     // group, counted in every timed launch in which the group was still
     // iterating (its PCG iteration count), averaged over the timed launches
     const double s = b->o.precision == DTC_FP64 ? 8.0 : 4.0;
-    double bytes = 0.0, flops = 0.0;
-    for (size_t i = 0; i < b->sb.h_elems.size() && i < b->iters.size(); ++i)
-        if (b->sb.h_status[i] == DTC_OK || b->sb.h_status[i] == DTC_ERR_NO_CONVERGENCE) {
-            const double ne = (double)b->sb.h_elems[i], nn = (double)b->sb.h_nodes[i], it = (double)b->iters[i];
-            bytes += it * (2.0 * (3.0 * nn * R * s) + 4.0 * ne);
-            flops += it * 1152.0 * R * ne;
-        }
+    for (const auto& S : b->subs) {
+        n += S->sb.mv_count;
+        ms += S->sb.mv_ms_sum;
+        for (size_t i = 0; i < S->sb.h_elems.size() && i < S->iters.size(); ++i)
+            if (S->sb.h_status[i] == DTC_OK || S->sb.h_status[i] == DTC_ERR_NO_CONVERGENCE) {
+                const double ne = (double)S->sb.h_elems[i], nn = (double)S->sb.h_nodes[i],
+                             it = (double)S->iters[i];
+                bytes += it * (2.0 * (3.0 * nn * R * s) + 4.0 * ne);
+                flops += it * 1152.0 * R * ne;
+            }
+    }
+    const double avg = n ? ms / (double)n : 0.0;
     if (mv_avg_ms) *mv_avg_ms = avg;
     if (mv_launches) *mv_launches = n;
     if (mv_bytes) *mv_bytes = n ? bytes / (double)n : 0.0;
@@ -627,7 +678,8 @@ int dtc_batch_stats(dtc_batch* b, double* mv_avg_ms, int64_t* mv_launches, doubl
 int dtc_batch_destroy(dtc_batch* b) {
     if (!b) return DTC_OK;
     if (b->st.s) (void)hipStreamSynchronize(b->st.s);
-    for (hipEvent_t e : b->sb.ev) (void)hipEventDestroy(e);
+    for (const auto& S : b->subs)
+        for (hipEvent_t e : S->sb.ev) (void)hipEventDestroy(e);
     delete b;
     return DTC_OK;
 }
@@ -649,7 +701,7 @@ int dtc_solve_domain(const dtc_domain_desc* d, const dtc_solve_opts* o, dtc_doma
         const int RM = macro_lcs(d->macro_order);
         const size_t col = (size_t)3 * res->n_nodes;
         for (int g = 0; g < b->ng[0] && rc == DTC_OK; ++g)
-            rc = batch_fields(b->sb, b->g0[0] + g, std::min(R, RM - R * g), u_out ? u_out + col * R * g : nullptr,
+            rc = batch_fields(b->subs[b->sbi[0]]->sb, b->g0[0] + g, std::min(R, RM - R * g), u_out ? u_out + col * R * g : nullptr,
                               r_out ? r_out + col * R * g : nullptr);
     }
     dtc_batch_destroy(b);

@@ -357,3 +357,27 @@ def test_mixed_macro_orders_in_one_batch():
         with dtc.Batch([d2], precision=1) as b:
             b.run()
             b.grad(0, np.eye(6))
+
+
+# ------------------------------------------------------------------ sub-batches
+def test_sub_batches_equal_one_batch_and_keep_the_sensitivity(monkeypatch):
+    """Batches are cut into sub-batches below a solver-node cap; results, statuses
+    and the sensitivity of a domain in an earlier sub-batch do not depend on it."""
+    dtc = _dtc()
+    from dtc_amd import synth
+    doms = [dtc.Domain(synth.trabecular(n, seed=90 + n, sigma=1.5, bvtv=0.4), delta=(0.01,) * 3)
+            for n in (10, 12, 9, 11)]
+    doms.insert(2, dtc.Domain(np.zeros((5, 5, 5), np.int8)))
+    G = np.eye(6)
+    with dtc.Batch(doms, precision=1, rel_tol=1e-8, rotation=False, precond=1) as b:
+        one = b.run()
+        g_one = b.grad(0, G)
+    monkeypatch.setenv("DTC_SUBBATCH_NODES", "3000")          # about two 10^3 domains per sub-batch
+    with dtc.Batch(doms, precision=1, rel_tol=1e-8, rotation=False, precond=1) as b:
+        sub = b.run()
+        g_sub = b.grad(0, G)
+    assert [r["status"] for r in sub] == [r["status"] for r in one] == [0, 0, 1, 0, 0]
+    for r1, r2 in zip(one, sub):
+        if r1["status"] == 0:
+            assert _rel(r2["C"], r1["C"]) < 1e-6
+    assert _rel(g_sub, g_one) < 1e-6
