@@ -264,31 +264,64 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
 
     // Only halo positions and owned positions that hold a node carry work: on
     // sparse (trabecular) domains a brick holds about a third of its 128
-    // positions.  Compact lists: s_hn = halo positions with a node (wave ballots
-    // and a prefix over the waves), s_ol = owned position of solver node
-    // node_begin + k (owned nodes are numbered in position order).
+    // positions.  s_ol = owned position of solver node node_begin + k (owned
+    // nodes are numbered in position order).  Staging, measured per precision:
+    //  * FP32 (EARLY_X false): compact list s_hn of halo positions with a node
+    //    (wave ballots, prefix over the waves); x loads issued after the barrier
+    //    for the existing (node, direction) items only;
+    //  * FP64 (EARLY_X true): x loads for every halo slot issued first, straight
+    //    from the brick table, before any barrier.
+    constexpr bool EARLY_X = sizeof(T) == 8;
     const int n_own = B.node_end - B.node_begin;
-    int nh = 0;
+    int nh = NH;
+    int pf_off[NLD], pf_xi[NLD];   // 16-byte offset in x and destination in s_x of this thread's items
+    V reg[NLD];                    // x segments of the next chunk to stage, loaded one chunk ahead
+    auto prefetch = [&](int chunk) {
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            V v;
+            vzero(v);
+            if (pf_off[k] >= 0) v = xv[pf_off[k] + chunk];
+            reg[k] = v;
+        }
+    };
+    if constexpr (EARLY_X) {
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            const int i = tid + k * MV_NT;
+            int o = -1;
+            if (IN == IN_V && i < NH * 3) {
+                const int q = halo[i / 3];
+                if (q >= 0) o = ((q & NODE_MASK) * 3 + i % 3) * NCH;
+            }
+            pf_off[k] = o;
+            pf_xi[k] = i;
+        }
+        if (IN == IN_V) prefetch(c0);
+    }
     {
         const int q = tid < NH ? halo[tid] : -1;
         if (tid < NH) s_node[tid] = q;
         const bool ex = q >= 0;
-        const unsigned long long bal = __ballot(ex);
-        const int lane = tid & 63, wv = tid >> 6;
-        if (lane == 0) s_wcnt[wv] = __popcll(bal);
         const int hx = tid % HX, hy = (tid / HX) % HY, hz = tid / (HX * HY);
         if (ex && hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ)
             s_ol[(q & NODE_MASK) - B.node_begin] = (unsigned char)((hx - 1) + BX * ((hy - 1) + BY * (hz - 1)));
         if (tid < NO) s_nm[tid] = A.nmask[(int64_t)blockIdx.x * NO + tid];
-        __syncthreads();
-        int base = 0;
+        if constexpr (!EARLY_X) {
+            const unsigned long long bal = __ballot(ex);
+            const int lane = tid & 63, wv = tid >> 6;
+            if (lane == 0) s_wcnt[wv] = __popcll(bal);
+            __syncthreads();
+            int base = 0;
+            nh = 0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            const int c = s_wcnt[w];
-            if (w < wv) base += c;
-            nh += c;
+            for (int w = 0; w < NW; ++w) {
+                const int c = s_wcnt[w];
+                if (w < wv) base += c;
+                nh += c;
+            }
+            if (ex) s_hn[base + __popcll(bal & ((1ull << lane) - 1))] = (short)tid;
         }
-        if (ex) s_hn[base + __popcll(bal & ((1ull << lane) - 1))] = (short)tid;
     }
     const int ne = B.e_cnt;
     {
@@ -301,33 +334,21 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
     }
     __syncthreads();
 
-    // staged (halo node, direction) items of this thread: i = tid + k * MV_NT < 3 nh,
-    // 16-byte offset in x and destination in s_x
-    int pf_off[NLD], pf_xi[NLD];
-#pragma unroll
-    for (int k = 0; k < NLD; ++k) {
-        const int i = tid + k * MV_NT;
-        int o = -1, xi = 0;
-        if (i < 3 * nh) {
-            const int h = s_hn[i / 3], d = i % 3;
-            o = ((s_node[h] & NODE_MASK) * 3 + d) * NCH;
-            xi = h * 3 + d;
-        }
-        pf_off[k] = o;
-        pf_xi[k] = xi;
-    }
-    // x segments of the next chunk to stage, loaded one chunk ahead
-    V reg[NLD];
-    auto prefetch = [&](int chunk) {
+    if constexpr (!EARLY_X) {
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
-            V v;
-            vzero(v);
-            if (pf_off[k] >= 0) v = xv[pf_off[k] + chunk];
-            reg[k] = v;
+            const int i = tid + k * MV_NT;
+            int o = -1, xi = 0;
+            if (i < 3 * nh) {
+                const int h = s_hn[i / 3], d = i % 3;
+                o = ((s_node[h] & NODE_MASK) * 3 + d) * NCH;
+                xi = h * 3 + d;
+            }
+            pf_off[k] = o;
+            pf_xi[k] = xi;
         }
-    };
-    if (IN == IN_V) prefetch(c0);
+        if (IN == IN_V) prefetch(c0);
+    }
 
     // s_x <- x of `chunk` (IN_V: from the prefetch registers; IN_U: x + BC values)
     auto stage = [&](int chunk) {
@@ -339,7 +360,7 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
             const DomInfo D = A.doms[B.dom];
             T* sx = reinterpret_cast<T*>(s_x);
             for (int i = tid; i < nh * SX; i += MV_NT) {
-                const int h = s_hn[i / SX], rem = i % SX, d = rem / RC, r = rem % RC;
+                const int h = EARLY_X ? i / SX : (int)s_hn[i / SX], rem = i % SX, d = rem / RC, r = rem % RC;
                 const int lc = chunk * RC + r;
                 const int q = s_node[h];
                 T v = T(0);
