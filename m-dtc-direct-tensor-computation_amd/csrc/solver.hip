@@ -88,6 +88,19 @@ __device__ inline void fht_inv(T v[8]) {
                 v[b + s] = a + c;
             }
 }
+// fht_inv after its first butterfly stage (s = 1)
+template <typename T>
+__device__ inline void fht_inv_tail(T v[8]) {
+#pragma unroll
+    for (int s = 2; s < 8; s <<= 1)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            if (!(b & s)) {
+                T a = v[b], c = v[b + s];
+                v[b] = a - c;
+                v[b + s] = a + c;
+            }
+}
 
 // y = Kw w on the (mode, direction) groups; w, y indexed [mode][dir]
 template <typename T>
@@ -248,7 +261,10 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
     __shared__ unsigned char s_nm[NO];
     __shared__ double s_wred[NCH][NW * 4][RC];   // per chunk, wave row and load case
     __shared__ V s_x[NH * 3];
-    __shared__ V s_c[(OB_BIAS + 8 * NO) * 3];   // corner buffers; the bias keeps element bases non-negative
+    // corner buffers (the bias keeps element bases non-negative), then a trash
+    // area for the outputs of corners outside the owned nodes (lane l writes
+    // vectors l .. l + 2; the contents are never read)
+    __shared__ V s_c[(OB_BIAS + 8 * NO) * 3 + 64 + 2];
     __shared__ short s_hn[NH];                   // halo positions that hold a node, compact
     __shared__ unsigned char s_ol[NO];           // owned position of the brick's k-th solver node
     __shared__ int s_wcnt[NW];
@@ -464,16 +480,27 @@ __global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T
                 T o[3][8];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
+                    // mode 0 (rigid translation) carries no force: the first
+                    // inverse butterfly of the pair (0, 1) reduces to (-y1, y1)
+                    const T y1 = y[1][d] * E;
+                    o[d][0] = -y1;
+                    o[d][1] = y1;
 #pragma unroll
-                    for (int b = 0; b < 8; ++b) o[d][b] = y[b][d] * E;
-                    fht_inv(o[d]);
-                }
-#pragma unroll
-                for (int b = 0; b < 8; ++b)
-                    if (mask & (1u << b)) {
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) cb[(b * NO + ocorner(b)) * SX + d * RC] = o[d][b];
+                    for (int b = 2; b < 8; b += 2) {
+                        const T a = y[b][d] * E, c = y[b + 1][d] * E;
+                        o[d][b] = a - c;
+                        o[d][b + 1] = a + c;
                     }
+                    fht_inv_tail(o[d]);
+                }
+                // corners outside the owned nodes write to this lane's trash slot
+                T* trash = sc + ((OB_BIAS + 8 * NO) * 3 + (tid & 63)) * RC;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    T* dst = (mask & (1u << b)) ? cb + (b * NO + ocorner(b)) * SX : trash;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) dst[d * RC] = o[d][b];
+                }
             }
         }
         __syncthreads();
