@@ -268,6 +268,7 @@ struct dtc_batch {
         std::vector<double> relres;
     };
     std::vector<std::unique_ptr<Sub>> subs;
+    int n_sub = 0;                 // sub-batches used by the last run
     std::vector<int> sbi, g0, ng;  // sub-batch, first solver group and group count of each domain
     double nu = 0.3, a0 = 1.0;
     std::vector<double> khat20;    // HEX20 micro: E = 1 element stiffness
@@ -505,8 +506,9 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         first.push_back(D);
     }
     const int NS = (int)first.size() - 1;
-    while ((int)b->subs.size() < NS) b->subs.emplace_back(new dtc_batch::Sub);
-    for (int k = 0; k < NS; ++k) {
+    // one sub-batch: mesh -> topology -> solve -> reactions for the listed domains
+    auto run_sub = [&](int k, const std::vector<int>& list, int precision) -> int {
+        while ((int)b->subs.size() <= k) b->subs.emplace_back(new dtc_batch::Sub);
         dtc_batch::Sub& S = *b->subs[k];
         SolverBatch& sb = S.sb;
         sb.stream = s;
@@ -516,13 +518,13 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         sb.khat20 = b->khat20;
         sb.timing = true;
         std::vector<dtc_domain_desc> gd;
-        for (int i = first[k]; i < first[k + 1]; ++i) {
+        for (int i : list) {
             b->sbi[i] = k;
             b->g0[i] = (int)gd.size();
             for (int g = 0; g < b->ng[i]; ++g) gd.push_back(b->d[i]);
         }
         DTC_TRY(batch_begin(sb, gd.data(), (int64_t)gd.size()));
-        for (int i = first[k]; i < first[k + 1]; ++i) {
+        for (int i : list) {
             const dtc_domain_desc& d = b->d[i];
             int rc = mesh_build_device(b->phi[i]->p, d.phi_kind, d.n, d.llimit, d.micro_nen, b->mesh, s);
             if (rc == DTC_ERR_EMPTY_MESH) {
@@ -536,15 +538,39 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
                 else DTC_TRY(batch_add_domain(sb, b->mesh, d, mod, b->escale[i], R * g, s));
             }
         }
-        DTC_TRY(batch_finalize(sb, b->o.precision));
+        DTC_TRY(batch_finalize(sb, precision));
         // the multigrid hierarchy coarsens the voxel lattice: HEX20 micro meshes use Jacobi
-        sb.precond = (b->o.precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG && !sb.micro20) ? DTC_PC_MG
-                                                                                                     : DTC_PC_JACOBI;
+        sb.precond = (precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG && !sb.micro20) ? DTC_PC_MG
+                                                                                               : DTC_PC_JACOBI;
         sb.mg_degree = b->o.mg_degree > 0 ? b->o.mg_degree : 2;
         if (sb.precond == DTC_PC_MG) DTC_TRY(batch_build_mg(sb, 8));
         DTC_TRY(batch_solve(sb, b->o, S.iters, S.outer, S.relres));
         DTC_TRY(batch_forces(sb, S.ff));
+        return DTC_OK;
+    };
+    for (int k = 0; k < NS; ++k) {
+        std::vector<int> list;
+        for (int i = first[k]; i < first[k + 1]; ++i) list.push_back(i);
+        DTC_TRY(run_sub(k, list, b->o.precision));
     }
+    // A mixed-precision solve that did not converge (a near-singular mode, e.g. parts
+    // hinged on one voxel edge, can grow without bound in the FP32 inner solve) is
+    // repeated for that domain in FP64 with Jacobi-PCG, the reference's own scheme.
+    int nsub = NS;
+    if (b->o.precision == DTC_FP32_REFINE) {
+        std::vector<int> retry;
+        for (int i = 0; i < D; ++i) {
+            const dtc_batch::Sub& S = *b->subs[b->sbi[i]];
+            bool bad = false;
+            for (int g = 0; g < b->ng[i]; ++g) bad = bad || S.sb.h_status[b->g0[i] + g] == DTC_ERR_NO_CONVERGENCE;
+            if (bad) retry.push_back(i);
+        }
+        if (!retry.empty()) {
+            DTC_TRY(run_sub(NS, retry, DTC_FP64));
+            nsub = NS + 1;
+        }
+    }
+    b->n_sub = nsub;
     b->EE.assign((size_t)D * 36, 0.0);
     std::vector<dtc_domain_result> out(D);
     std::vector<int> solved;
@@ -654,8 +680,9 @@ int dtc_batch_stats(dtc_batch* b, double* mv_avg_ms, int64_t* mv_launches, doubl
     // B_mv = 2 (3 N_n R s) + 4 N_e, F_mv = 1152 R N_e (dense-equivalent) per domain
     // group, counted in every timed launch in which the group was still
     // iterating (its PCG iteration count), averaged over the timed launches
-    const double s = b->o.precision == DTC_FP64 ? 8.0 : 4.0;
-    for (const auto& S : b->subs) {
+    for (int k = 0; k < b->n_sub && k < (int)b->subs.size(); ++k) {
+        const auto& S = b->subs[k];
+        const double s = S->sb.precision == DTC_FP64 ? 8.0 : 4.0;
         n += S->sb.mv_count;
         ms += S->sb.mv_ms_sum;
         for (size_t i = 0; i < S->sb.h_elems.size() && i < S->iters.size(); ++i)

@@ -2147,17 +2147,55 @@ int residual64(SolverBatch& b, const WK<double>& wk, const double* x64, int firs
     return DTC_OK;
 }
 
-// inner iteration cap reached for some domains: mark them finished
-int apply_cap(SolverBatch& b, int64_t cap, std::vector<int64_t>& nouter, bool* any_left) {
+// Progress of the FP64 true residual over the residual replacements of the mixed
+// solve: a domain whose worst load case has not halved its relative residual in
+// STALL replacements, or whose residual is not finite, is given up (the caller
+// repeats it in FP64).
+struct Progress {
+    static constexpr int STALL = 10;
+    std::vector<double> best;
+    std::vector<int64_t> at;
+};
+
+// inner iteration cap reached (or no progress, see Progress) for some domains:
+// mark them finished
+int apply_cap(SolverBatch& b, int64_t cap, std::vector<int64_t>& nouter, bool* any_left, Progress* pg = nullptr) {
     const int D = b.n_dom;
     std::vector<int> h(D);
     std::vector<int64_t> hi(D);
+    std::vector<double> hrr, hr0;
     DTC_HIP(hipMemcpyAsync(h.data(), b.done64.p, (size_t)D * 4, hipMemcpyDeviceToHost, b.stream));
     DTC_HIP(hipMemcpyAsync(hi.data(), b.iters.p, (size_t)D * 8, hipMemcpyDeviceToHost, b.stream));
+    if (pg) {
+        hrr.resize((size_t)D * R);
+        hr0.resize((size_t)D * R);
+        DTC_HIP(hipMemcpyAsync(hrr.data(), b.rr.p, hrr.size() * 8, hipMemcpyDeviceToHost, b.stream));
+        DTC_HIP(hipMemcpyAsync(hr0.data(), b.rr0.p, hr0.size() * 8, hipMemcpyDeviceToHost, b.stream));
+        if ((int)pg->best.size() != D) {
+            pg->best.assign(D, -1.0);
+            pg->at.assign(D, 0);
+        }
+    }
     DTC_HIP(hipStreamSynchronize(b.stream));
     bool any = false, changed = false;
     for (int d = 0; d < D; ++d) {
-        if (!h[d] && hi[d] >= cap) { h[d] = 1; changed = true; }
+        bool stop = !h[d] && hi[d] >= cap;
+        if (pg && !h[d] && !stop) {
+            double worst = 0.0;
+            for (int lc = 0; lc < R; ++lc) {
+                const double r0 = hr0[(size_t)d * R + lc], rr = hrr[(size_t)d * R + lc];
+                const double v = r0 > 0.0 ? std::sqrt(rr / r0) : 0.0;
+                worst = std::isfinite(v) && std::isfinite(r0) ? std::fmax(worst, v) : INFINITY;
+            }
+            if (!std::isfinite(worst)) stop = true;
+            else if (pg->best[d] < 0.0 || worst < 0.5 * pg->best[d]) {
+                pg->best[d] = worst;
+                pg->at[d] = nouter[d];
+            } else if (nouter[d] - pg->at[d] >= Progress::STALL) {
+                stop = true;
+            }
+        }
+        if (stop) { h[d] = 1; changed = true; }
         if (!h[d]) { nouter[d] += 1; any = true; }
     }
     if (changed) {
@@ -2227,6 +2265,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
         // replaced by the true FP64 residual and the search direction p32 is
         // kept.
         const bool mg = b.precond == DTC_PC_MG && !b.micro20;
+        Progress progress;   // domains that stop converging are given up (the caller repeats them in FP64)
         const int64_t REPLACE = mg ? 100 : 1000;
         float* x32 = b.x32.as<float>();
         float* r32 = b.r32.as<float>();
@@ -2235,7 +2274,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
         bool first = true;
         for (int it = 0; it < 100000; ++it) {
             bool left = false;
-            DTC_TRY(apply_cap(b, cap, nouter, &left));
+            DTC_TRY(apply_cap(b, cap, nouter, &left, &progress));
             if (!left) break;
             k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.r64.as<double>(), r32, x32);
             if (mg) DTC_TRY(mg_vcycle(b, wk32, r32, z32, nullptr));
