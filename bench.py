@@ -80,8 +80,8 @@ def cpu_baseline(dom, gpu_fp64_iters_per_lc, sample_iters, log):
 
 
 def uneven(args, log):
-    """A batch of uneven synthetic domains solved in one dtc_batch_run (split into
-    sub-batches by the library), domains/s over the whole batch."""
+    """Uneven synthetic domains solved in runs of 128 (each one dtc_batch_run, split
+    into sub-batches by the library), domains/s over all runs."""
     import dtc_amd
     from dtc_amd import synth
     t0 = time.perf_counter()
@@ -89,13 +89,16 @@ def uneven(args, log):
     log(f"generated {len(phis)} domains in {time.perf_counter() - t0:.1f} s; "
         f"{sum(p.size for p in phis) / 1e6:.1f} M voxels")
     doms = [dtc_amd.Domain(p, llimit=1, delta=(0.01, 0.01, 0.01), xa_n=(2, 2, 2)) for p in phis]
-    with dtc_amd.Batch(doms, precision=dtc_amd.DTC_FP32_REFINE, rel_tol=1e-8, rotation=True, rot_step_deg=5.0,
-                       precond=dtc_amd.DTC_PC_MG, max_iter=args.max_iter) as b:
-        _sync()
-        t0 = time.perf_counter()
-        res = b.run()
-        _sync()
-        dt = time.perf_counter() - t0
+    res, dt, G = [], 0.0, 128      # runs of 128 domains (each split into sub-batches by the library)
+    for g0 in range(0, len(doms), G):
+        with dtc_amd.Batch(doms[g0:g0 + G], precision=dtc_amd.DTC_FP32_REFINE, rel_tol=1e-8, rotation=True,
+                           rot_step_deg=5.0, precond=dtc_amd.DTC_PC_MG, max_iter=args.max_iter) as b:
+            _sync()
+            t0 = time.perf_counter()
+            res += b.run()
+            _sync()
+            dt += time.perf_counter() - t0
+        log(f"domains {g0}..{min(g0 + G, len(doms)) - 1}: {dt:.1f} s so far")
     ok = sum(1 for r in res if r["status"] == 0 and r["relres"] <= 1e-8)
     st = {}
     for i, r in enumerate(res):
