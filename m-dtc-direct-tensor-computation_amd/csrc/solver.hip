@@ -248,8 +248,10 @@ constexpr int CH_READ_D = 1, CH_D_FROM_X = 2, CH_WRITE_D = 4;
 // Dirichlet rows; OUT_CHEB applies one Chebyshev-Jacobi step to the owned nodes,
 //   d = c1 d + c2 D^-1 (b - y),  yout = x + d   (yout must not alias x),
 // and, when partial is set, writes per-brick sums of b . yout.
+// launch bounds: at least 4 waves per SIMD (2 workgroups of 8 waves per CU), which
+// caps the kernel at 128 VGPRs
 template <typename T, int IN, int OUT>
-__global__ __launch_bounds__(MV_NT, 2) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
+__global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
                                                  T* __restrict__ yout) {
     constexpr int RC = MvCfg<T>::RC, SX = 3 * RC, NCH = R / RC;
     using V = typename MvCfg<T>::V;
