@@ -27,6 +27,7 @@
 //    r = -(K u)_free and the reactions in FP64.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
@@ -250,7 +251,8 @@ constexpr int CH_READ_D = 1, CH_D_FROM_X = 2, CH_WRITE_D = 4;
 // and, when partial is set, writes per-brick sums of b . yout.
 // launch bounds: at least 4 waves per SIMD (2 workgroups of 8 waves per CU), which
 // caps the kernel at 128 VGPRs
-template <typename T, int IN, int OUT>
+// IO32 (T = double, IN_V only): x and y are FP32 vectors, the arithmetic is FP64
+template <typename T, int IN, int OUT, bool IO32 = false>
 __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
                                                  T* __restrict__ yout) {
     constexpr int RC = MvCfg<T>::RC, SX = 3 * RC, NCH = R / RC;
@@ -271,6 +273,7 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
     __shared__ unsigned char s_ol[NO];           // owned position of the brick's k-th solver node
     __shared__ int s_wcnt[NW];
     static_assert(NH <= MV_NT, "one thread per halo position");
+    static_assert(!IO32 || (sizeof(T) == 8 && IN == IN_V), "FP32 vectors with FP64 arithmetic");
 
     const BrickInfo B = A.bricks[blockIdx.x];
     if (A.done && A.done[B.dom]) return;
@@ -299,7 +302,14 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
         for (int k = 0; k < NLD; ++k) {
             V v;
             vzero(v);
-            if (pf_off[k] >= 0) v = xv[pf_off[k] + chunk];
+            if (pf_off[k] >= 0) {
+                if constexpr (IO32) {
+                    const float2 f = reinterpret_cast<const float2*>(xin)[pf_off[k] + chunk];
+                    v = make_double2((double)f.x, (double)f.y);
+                } else {
+                    v = xv[pf_off[k] + chunk];
+                }
+            }
             reg[k] = v;
         }
     };
@@ -571,7 +581,8 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
                         for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(bv, w) * (double)vcomp(y, w);
                 }
             }
-            yv[vi] = y;
+            if constexpr (IO32) reinterpret_cast<float2*>(yout)[vi] = make_float2((float)y.x, (float)y.y);
+            else yv[vi] = y;
         }
         if (OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) {
             // every lane holds the RC load cases of the chunk: fixed-order row sums
@@ -823,28 +834,334 @@ __global__ __launch_bounds__(RED_NT) void k_red_outer(const DomInfo* __restrict_
     }
 }
 
-// r32 = r64, x32 = 0 (residual replacement)
+// column scale of the FP32 correction problem: 1 / ||r64|| of the load case
+// (rr = ||r64||^2 per load case), 1 without scaling or for a zero residual
+__device__ inline double col_scale(const double* __restrict__ rr, int dom, int lc) {
+    if (!rr) return 1.0;
+    const double v = rr[dom * R + lc];
+    return v > 0.0 ? 1.0 / sqrt(v) : 1.0;
+}
+
+// r32 = r64 (times its column scale when rr is given), x32 = 0 (residual replacement)
 __global__ __launch_bounds__(NT_UPD) void k_to32(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
                                                  const double* __restrict__ r64, float* __restrict__ r32,
-                                                 float* __restrict__ x32) {
+                                                 float* __restrict__ x32, const double* __restrict__ rr) {
     const BrickInfo B = bricks[blockIdx.x];
     if (done[B.dom]) return;
+    const double sc = col_scale(rr, B.dom, threadIdx.x % R);      // NT_UPD is a multiple of 24
     const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
     for (int64_t i = threadIdx.x; i < cnt; i += NT_UPD) {
-        r32[base + i] = (float)r64[base + i];
+        r32[base + i] = (float)(r64[base + i] * sc);
         x32[base + i] = 0.f;
     }
 }
 
-// x64 += x32, x32 = 0 (group update of the FP32 correction)
+// x64 += x32 (divided by its column scale when rr is given), x32 = 0 (group
+// update of the FP32 correction)
 __global__ __launch_bounds__(NT_UPD) void k_acc64(const BrickInfo* __restrict__ bricks, const int* __restrict__ skip,
-                                                  float* __restrict__ x32, double* __restrict__ x64) {
+                                                  float* __restrict__ x32, double* __restrict__ x64,
+                                                  const double* __restrict__ rr) {
     const BrickInfo B = bricks[blockIdx.x];
     if (skip[B.dom]) return;
+    const double sc = 1.0 / col_scale(rr, B.dom, threadIdx.x % R);
     const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
     for (int64_t i = threadIdx.x; i < cnt; i += NT_UPD) {
-        x64[base + i] += (double)x32[base + i];
+        x64[base + i] += (double)x32[base + i] * sc;
         x32[base + i] = 0.f;
+    }
+}
+
+// ---------------------------------------------------------------- block PCG
+// The 24 load cases of a domain share one Krylov block: block PCG in the
+// A-orthogonal form
+//   Q = A P,  W = P^T Q,  alpha = W^+ (P^T R),  X += P alpha,  R -= Q alpha,
+//   Z = M R,  beta = -W^+ (Q^T Z),  P = Z + P beta.
+// A vector row is one (node, direction) pair holding the 24 load cases.
+constexpr int GR_NT = 256;          // 4 waves
+constexpr int SLICE_NODES = 512;    // nodes per Gram workgroup (at least)
+constexpr int MAX_SLICES = 1024;    // Gram workgroups per domain (at most)
+constexpr int RR = R * R;           // entries of a 24 x 24 coefficient matrix
+constexpr double PIVOT_DROP = 1e-9;   // of P^T A P
+constexpr double ORTH_DROP = 1e-10;   // of P^T P
+
+// Gram partial sums over the rows of one slice of a domain's nodes:
+//   g[0] = A^T B, g[1] = A^T C (NGR = 2; zero for NGR = 1), FP64 sums of FP32
+// products.  Lane (i, j) of a wave keeps rows 3i..3i+2 x columns 3j..3j+2; the
+// four waves take every fourth vector row and are summed in a fixed order.
+template <int NGR>
+__global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices, const int* __restrict__ done,
+                                               const float* __restrict__ A, const float* __restrict__ B,
+                                               const float* __restrict__ C, double* __restrict__ gpart) {
+    __shared__ double s_g[4][2 * RR];
+    const int4 sl = slices[blockIdx.x];   // domain, first node, end node
+    if (done[sl.x]) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ci = 3 * (lane >> 3), cj = 3 * (lane & 7);
+    double g[NGR][9];
+#pragma unroll
+    for (int t = 0; t < NGR; ++t)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) g[t][k] = 0.0;
+    // GU rows per wave and step, their loads issued together (a slice is a long
+    // sequential walk: one load round trip per step)
+    constexpr int GU = 2;
+    const int64_t r1 = (int64_t)sl.z * 3;
+    for (int64_t row0 = (int64_t)sl.y * 3 + wave; row0 < r1; row0 += 4 * GU) {
+        float av[GU][3], bv[NGR][GU][3];
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            const int64_t row = row0 + 4 * u;
+            const bool in = row < r1;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                av[u][i] = in ? A[row * R + ci + i] : 0.f;
+                bv[0][u][i] = in ? B[row * R + cj + i] : 0.f;
+                if (NGR == 2) bv[NGR - 1][u][i] = in ? C[row * R + cj + i] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            double ad[3], bd[NGR][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                ad[i] = (double)av[u][i];
+#pragma unroll
+                for (int t = 0; t < NGR; ++t) bd[t][i] = (double)bv[t][u][i];
+            }
+#pragma unroll
+            for (int t = 0; t < NGR; ++t)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) g[t][3 * i + j] = fma(ad[i], bd[t][j], g[t][3 * i + j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            s_g[wave][(ci + i) * R + cj + j] = g[0][3 * i + j];
+            s_g[wave][RR + (ci + i) * R + cj + j] = NGR == 2 ? g[NGR - 1][3 * i + j] : 0.0;
+        }
+    __syncthreads();
+    double* out = gpart + (int64_t)blockIdx.x * 2 * RR;
+    for (int e = tid; e < 2 * RR; e += GR_NT) out[e] = ((s_g[0][e] + s_g[1][e]) + s_g[2][e]) + s_g[3][e];
+}
+
+// gsum[d] = sum over the domain's slices of the Gram partials, in a fixed order:
+// workgroup (d, c) sums entries 64c..64c+63 of the 2 x 576; wave w takes the
+// slices w, w + 4, ... and the four waves are added in order
+__global__ __launch_bounds__(256) void k_gram_sum(const int* __restrict__ dslice, const int* __restrict__ done,
+                                                 const double* __restrict__ gpart, double* __restrict__ gsum) {
+    __shared__ double s_w[4][64];
+    const int d = blockIdx.x;
+    if (done[d]) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, e = (int)blockIdx.y * 64 + lane;
+    const int s0 = dslice[d], s1 = dslice[d + 1];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    int s = s0 + wave;
+    for (; s + 12 < s1; s += 16)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] += gpart[(int64_t)(s + 4 * u) * 2 * RR + e];
+    for (int u = 0; s < s1; s += 4, ++u) a[u & 3] += gpart[(int64_t)s * 2 * RR + e];
+    s_w[wave][lane] = ((a[0] + a[1]) + a[2]) + a[3];
+    __syncthreads();
+    if (wave == 0) gsum[(int64_t)d * 2 * RR + e] = ((s_w[0][lane] + s_w[1][lane]) + s_w[2][lane]) + s_w[3][lane];
+}
+
+// Per domain: takes the Gram sums of k_gram_sum, then
+//   mode 0: W = sym(g[0]) = P^T A P is factored as L L^T and L kept; a pivot at
+//           or below 1e-9 of its diagonal entry before elimination (or not
+//           finite) marks a direction that depends on the previous ones: its
+//           column of L is zero and the solves give it a zero coefficient.
+//           coef = W^+ g[1];
+//   mode 1: coef = -W^+ g[0] with the kept factor;
+//   mode 2: G = sym(g[0]) = P^T P is factored the same way (pivots at or below
+//           1e-10 dropped, L not kept) and coef = L^-T, so that P coef has
+//           orthonormal columns (zero columns for the dropped directions).
+// coef[k * 24 + j] multiplies vector column k in the update of column j.
+constexpr int BS_NT = 256;
+
+__global__ __launch_bounds__(BS_NT) void k_bsolve(const int* __restrict__ done, const double* __restrict__ gsum,
+                                                 int mode, double* __restrict__ lfac, float* __restrict__ coef) {
+    __shared__ double s_l[RR], s_b[RR], s_d[R];
+    const int d = blockIdx.x, lane = threadIdx.x;
+    if (done[d]) return;
+    double* L = lfac + (int64_t)d * RR;
+    for (int e = lane; e < RR; e += BS_NT) {
+        const double w = gsum[(int64_t)d * 2 * RR + e];
+        const double g = mode == 0 ? gsum[(int64_t)d * 2 * RR + RR + e] : 0.0;
+        if (mode == 0) {
+            s_l[e] = w;
+            s_b[e] = g;
+        } else if (mode == 1) {
+            s_b[e] = w;
+            s_l[e] = L[e];
+        } else {
+            s_l[e] = w;
+            s_b[e] = (e / R == e % R) ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+    if (mode != 1) {
+        const double drop = mode == 0 ? PIVOT_DROP : ORTH_DROP;
+        constexpr int NV = (RR + BS_NT - 1) / BS_NT;
+        double v[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            const int e = lane + BS_NT * n, i = e / R, j = e % R;
+            if (e < RR) v[n] = 0.5 * (s_l[i * R + j] + s_l[j * R + i]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int n = 0; n < NV; ++n)
+            if (lane + BS_NT * n < RR) s_l[lane + BS_NT * n] = v[n];
+        __syncthreads();
+        if (lane < R) s_d[lane] = s_l[lane * (R + 1)];
+        __syncthreads();
+        for (int k = 0; k < R; ++k) {
+            const double dk = s_l[k * R + k];
+            const bool keep = dk > drop * s_d[k] && dk < INFINITY;
+            const double lk = keep ? sqrt(dk) : 0.0;
+            __syncthreads();
+            if (lane >= k && lane < R) s_l[lane * R + k] = keep ? (lane == k ? lk : s_l[lane * R + k] / lk) : 0.0;
+            __syncthreads();
+            for (int e = lane; e < RR; e += BS_NT) {
+                const int i = e / R, j = e % R;
+                if (j > k && i >= j) s_l[e] -= s_l[i * R + k] * s_l[j * R + k];
+            }
+            __syncthreads();
+        }
+        if (mode == 0)
+            for (int e = lane; e < RR; e += BS_NT) L[e] = s_l[e];
+    }
+    __syncthreads();
+    // L L^T y = b (modes 0, 1) or L^T y = I (mode 2) for column j of b, in place
+    // (lane j owns column j of s_b)
+    if (lane < R) {
+        const int j = lane;
+        for (int k = 0; k < R && mode != 2; ++k) {
+            double t = s_b[k * R + j];
+            for (int m = 0; m < k; ++m) t -= s_l[k * R + m] * s_b[m * R + j];
+            const double lkk = s_l[k * R + k];
+            s_b[k * R + j] = lkk > 0.0 ? t / lkk : 0.0;
+        }
+        for (int k = R - 1; k >= 0; --k) {
+            double t = s_b[k * R + j];
+            for (int m = k + 1; m < R; ++m) t -= s_l[m * R + k] * s_b[m * R + j];
+            const double lkk = s_l[k * R + k];
+            s_b[k * R + j] = lkk > 0.0 ? t / lkk : 0.0;
+        }
+        for (int k = 0; k < R; ++k)
+            coef[(int64_t)d * RR + k * R + j] = (float)(mode == 1 ? -s_b[k * R + j] : s_b[k * R + j]);
+    }
+}
+
+// X += P alpha, R -= Q alpha on the rows of each brick (alpha: k_bsolve mode 0),
+// and per-brick sums of r.r per load case.  Of the 6 threads of a row, thread q
+// keeps load cases 4q..4q+3.
+__global__ __launch_bounds__(NT_UPD) void k_bupdate(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                                   const float* __restrict__ coef, float* __restrict__ x,
+                                                   float* __restrict__ r, const float* __restrict__ p,
+                                                   const float* __restrict__ q, double* __restrict__ prr) {
+    __shared__ float4 s_c[R * 6];
+    __shared__ double s_a[NT_UPD][4];
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done[B.dom]) return;
+    const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
+    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)B.dom * RR);
+    for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
+    __syncthreads();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t r1 = (int64_t)B.node_end * 3;
+    for (int64_t row = (int64_t)B.node_begin * 3 + rw; row < r1; row += NT_UPD / 6) {
+        const float4* pr = reinterpret_cast<const float4*>(p + row * R);
+        const float4* qr = reinterpret_cast<const float4*>(q + row * R);
+        float4* xr = reinterpret_cast<float4*>(x + row * R) + qd;
+        float4* rr = reinterpret_cast<float4*>(r + row * R) + qd;
+        float4 xv = *xr, rv = *rr;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const float4 pv = pr[c], qv = qr[c];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const float4 a = s_c[(4 * c + w) * 6 + qd];
+                const float pk = vcomp(pv, w), qk = vcomp(qv, w);
+                xv.x += pk * a.x; xv.y += pk * a.y; xv.z += pk * a.z; xv.w += pk * a.w;
+                rv.x -= qk * a.x; rv.y -= qk * a.y; rv.z -= qk * a.z; rv.w -= qk * a.w;
+            }
+        }
+        *xr = xv;
+        *rr = rv;
+        acc[0] += (double)rv.x * (double)rv.x;
+        acc[1] += (double)rv.y * (double)rv.y;
+        acc[2] += (double)rv.z * (double)rv.z;
+        acc[3] += (double)rv.w * (double)rv.w;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s_a[tid][w] = acc[w];
+    __syncthreads();
+    if (tid < R) {
+        const int qq = tid / 4, w = tid % 4;
+        double s = 0.0;
+        for (int k = 0; k < NT_UPD / 6; ++k) s += s_a[k * 6 + qq][w];
+        prr[(int64_t)blockIdx.x * R + tid] = s;
+    }
+}
+
+// pn = Z + P coef (z null: pn = P coef); pn aliases neither p nor z.  With beta
+// (k_bsolve mode 1) the new direction block, with L^-T (mode 2) its orthonormal
+// columns
+__global__ __launch_bounds__(NT_UPD) void k_bdir(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                                const float* __restrict__ coef, const float* __restrict__ p,
+                                                const float* __restrict__ z, float* __restrict__ pn) {
+    __shared__ float4 s_c[R * 6];
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done[B.dom]) return;
+    const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
+    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)B.dom * RR);
+    for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
+    __syncthreads();
+    const int64_t r1 = (int64_t)B.node_end * 3;
+    for (int64_t row = (int64_t)B.node_begin * 3 + rw; row < r1; row += NT_UPD / 6) {
+        const float4* pr = reinterpret_cast<const float4*>(p + row * R);
+        float4 v = z ? reinterpret_cast<const float4*>(z + row * R)[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const float4 pv = pr[c];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const float4 a = s_c[(4 * c + w) * 6 + qd];
+                const float pk = vcomp(pv, w);
+                v.x += pk * a.x; v.y += pk * a.y; v.z += pk * a.z; v.w += pk * a.w;
+            }
+        }
+        reinterpret_cast<float4*>(pn + row * R)[qd] = v;
+    }
+}
+
+// per domain: the inner block iteration has finished when every load case's
+// recursive residual (scaled back by its column scale, see k_to32) is at or
+// below its target (a non-finite one is not), or at the iteration cap
+__global__ __launch_bounds__(RED_NT) void k_bcheck(const DomInfo* __restrict__ doms, int* __restrict__ done,
+                                                  const double* __restrict__ prr, const double* __restrict__ thr2,
+                                                  const double* __restrict__ rrcur, int64_t* __restrict__ iters,
+                                                  int64_t max_iters) {
+    __shared__ double s_out[4 * R];
+    const int d = blockIdx.x, tid = threadIdx.x;
+    if (done[d]) return;
+    const DomInfo D = doms[d];
+    double rr[R];
+    dom_sum24(prr, D.brick_begin, D.brick_end, s_out, rr);
+    if (tid == 0) {
+        int any = 0;
+        for (int lc = 0; lc < R; ++lc) {
+            const double sc = col_scale(rrcur, d, lc);
+            any |= !(rr[lc] <= thr2[d * R + lc] * sc * sc);
+        }
+        const int64_t it = iters[d] + 1;
+        iters[d] = it;
+        if (!any || it >= max_iters) done[d] = 1;
     }
 }
 
@@ -1930,8 +2247,9 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
     if (b.n_bricks == 0) return DTC_OK;
     MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(), b.einfo.as<int>(),
              sizeof(T) == 4 ? b.eE32.p : b.eE64.p, done, part};
-    // the CG matvec is the dominant kernel: bracket it with events on the batch stream
-    const bool t = b.timing && IN == IN_V && OUT == OUT_MASKED_DOT;
+    // the CG matvec (OUT_MASKED: of the block PCG) is the dominant kernel: bracket it
+    // with events on the batch stream
+    const bool t = b.timing && IN == IN_V && (OUT == OUT_MASKED_DOT || OUT == OUT_MASKED);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (t) {
         DTC_TRY(event_pair(b, &e0, &e1));
@@ -2136,6 +2454,121 @@ int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* di
     return DTC_OK;
 }
 
+// Q = A P of the block PCG: FP32 vectors, FP64 arithmetic.  The Gram matrix
+// W = P^T Q resolves the directions of the block only as far as Q is accurate:
+// for smooth directions FP32 sums lose most digits of A P to cancellation, and
+// W's small eigenvalues (the ones that steer the block) drown in that error.
+int launch_mv_q64(SolverBatch& b, const float* p, float* q, const int* done) {
+    MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(),
+             b.einfo.as<int>(), b.eE64.p, done, nullptr};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (b.timing) {
+        DTC_TRY(event_pair(b, &e0, &e1));
+        DTC_HIP(hipEventRecord(e0, b.stream));
+    }
+    const dim3 grid(b.n_bricks, chunk_split(b.n_bricks, R / MvCfg<double>::RC));
+    k_matvec<double, IN_V, OUT_MASKED, true><<<grid, MV_NT, 0, b.stream>>>(
+        a, cast_wk<double>(b.wk64), reinterpret_cast<const double*>(p), reinterpret_cast<double*>(q));
+    DTC_HIP(hipGetLastError());
+    if (b.timing) DTC_HIP(hipEventRecord(e1, b.stream));
+    return DTC_OK;
+}
+
+// block PCG set-up: slices of SLICE_NODES nodes (more for domains of over
+// MAX_SLICES * SLICE_NODES nodes) within each domain for the Gram sums, and the
+// per-domain factor and coefficient storage
+int block_setup(SolverBatch& b) {
+    if (b.n_slices > 0) return DTC_OK;
+    std::vector<int4> sl;
+    std::vector<int> ds((size_t)b.n_dom + 1);
+    for (int d = 0; d < b.n_dom; ++d) {
+        ds[d] = (int)sl.size();
+        const DomInfo& D = b.h_dom[d];
+        const int64_t nn = D.node_end - D.node_begin;
+        const int64_t step = std::max<int64_t>(SLICE_NODES, (nn + MAX_SLICES - 1) / MAX_SLICES);
+        for (int64_t n0 = D.node_begin; n0 < D.node_end; n0 += step)
+            sl.push_back(make_int4(d, (int)n0, (int)std::min<int64_t>(n0 + step, D.node_end), 0));
+    }
+    ds[b.n_dom] = (int)sl.size();
+    DTC_TRY(b.slices.alloc(std::max<size_t>(sl.size(), 1) * sizeof(int4)));
+    DTC_TRY(b.dslice.alloc(ds.size() * 4));
+    DTC_TRY(b.gpart.alloc(std::max<size_t>(sl.size(), 1) * 2 * RR * 8));
+    DTC_TRY(b.gsum.alloc((size_t)std::max(b.n_dom, 1) * 2 * RR * 8));
+    DTC_TRY(b.lfac.alloc((size_t)std::max(b.n_dom, 1) * RR * 8));
+    DTC_TRY(b.bcoef.alloc((size_t)std::max(b.n_dom, 1) * RR * 4));
+    if (!sl.empty())
+        DTC_HIP(hipMemcpyAsync(b.slices.p, sl.data(), sl.size() * sizeof(int4), hipMemcpyHostToDevice, b.stream));
+    DTC_HIP(hipMemcpyAsync(b.dslice.p, ds.data(), ds.size() * 4, hipMemcpyHostToDevice, b.stream));
+    DTC_HIP(hipStreamSynchronize(b.stream));
+    b.n_slices = (int)sl.size();
+    return DTC_OK;
+}
+
+// per-domain Grams a^T b and a^T c (c may be null) into gsum
+int block_gram(SolverBatch& b, const float* a, const float* bb, const float* c) {
+    hipStream_t s = b.stream;
+    int* done = b.done.as<int>();
+    if (b.n_slices > 0 && c)
+        k_gram<2><<<b.n_slices, GR_NT, 0, s>>>(b.slices.as<int4>(), done, a, bb, c, b.gpart.as<double>());
+    else if (b.n_slices > 0)
+        k_gram<1><<<b.n_slices, GR_NT, 0, s>>>(b.slices.as<int4>(), done, a, bb, nullptr, b.gpart.as<double>());
+    k_gram_sum<<<dim3(b.n_dom, 2 * RR / 64), 256, 0, s>>>(b.dslice.as<int>(), done, b.gpart.as<double>(),
+                                                         b.gsum.as<double>());
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+// dst = src with orthonormal columns per domain (dropped directions zero).  In
+// FP32, combinations of nearly dependent columns with large coefficients that
+// cancel lose the directions they should keep: the residual columns of a
+// domain line up with its slowest modes after the first residual replacement.
+int block_orth(SolverBatch& b, const float* src, float* dst) {
+    hipStream_t s = b.stream;
+    int* done = b.done.as<int>();
+    DTC_TRY(block_gram(b, src, src, nullptr));
+    k_bsolve<<<b.n_dom, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 2, b.lfac.as<double>(), b.bcoef.as<float>());
+    k_bdir<<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), done, b.bcoef.as<float>(), src, nullptr, dst);
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+// one block PCG iteration on (x, r); p is the direction block (orthonormal
+// columns), q and z are work vectors
+int block_iteration(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p, float* q, float* z,
+                    int64_t max_iters) {
+    hipStream_t s = b.stream;
+    const int D = b.n_dom, NBk = b.n_bricks;
+    int* done = b.done.as<int>();
+    const BrickInfo* bl = b.brick.as<BrickInfo>();
+    if (b.block_q64) DTC_TRY(launch_mv_q64(b, p, q, done));
+    else DTC_TRY((launch_mv<float, IN_V, OUT_MASKED>(b, wk, p, q, nullptr, done)));
+    DTC_TRY(block_gram(b, p, q, r));
+    k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 0, b.lfac.as<double>(), b.bcoef.as<float>());
+    k_bupdate<<<NBk, NT_UPD, 0, s>>>(bl, done, b.bcoef.as<float>(), x, r, p, q, b.part_b.as<double>());
+    k_bcheck<<<D, RED_NT, 0, s>>>(b.dom.as<DomInfo>(), done, b.part_b.as<double>(), b.thr2.as<double>(),
+                                  b.rr.as<double>(), b.iters.as<int64_t>(), max_iters);
+    DTC_HIP(hipGetLastError());
+    DTC_TRY(mg_vcycle(b, wk, r, z, nullptr));
+    DTC_TRY(block_gram(b, q, z, nullptr));
+    k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 1, b.lfac.as<double>(), b.bcoef.as<float>());
+    k_bdir<<<NBk, NT_UPD, 0, s>>>(bl, done, b.bcoef.as<float>(), p, z, q);
+    DTC_HIP(hipGetLastError());
+    return block_orth(b, q, p);
+}
+
+// block PCG iterations until every domain is done or max_steps iterations ran
+int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p, float* q, float* z,
+              int64_t max_iters, int64_t max_steps) {
+    const int CHECK = 4;
+    for (int64_t steps = 0; steps < max_steps; steps += CHECK) {
+        bool fin = false;
+        DTC_TRY(all_done(b, b.done, &fin));
+        if (fin) return DTC_OK;
+        for (int k = 0; k < CHECK; ++k) DTC_TRY(block_iteration(b, wk, x, r, p, q, z, max_iters));
+    }
+    return DTC_OK;
+}
+
 // FP64 true residual r64 = -(K u)_free, u = x64 + x_bc (x64 == null: u = x_bc),
 // then per load case activity and targets (k_red_outer)
 int residual64(SolverBatch& b, const WK<double>& wk, const double* x64, int first, const dtc_solve_opts& o) {
@@ -2179,6 +2612,21 @@ int apply_cap(SolverBatch& b, int64_t cap, std::vector<int64_t>& nouter, bool* a
         }
     }
     DTC_HIP(hipStreamSynchronize(b.stream));
+    // DTC_TRACE=1: one stderr line per residual replacement of the mixed solve
+    // (domain: PCG iterations so far / worst relative FP64 residual)
+    static const bool trace = std::getenv("DTC_TRACE") && std::getenv("DTC_TRACE")[0] == '1';
+    if (trace && pg) {
+        std::fprintf(stderr, "[dtc] replacement:");
+        for (int d = 0; d < D; ++d) {
+            double worst = 0.0;
+            for (int lc = 0; lc < R; ++lc) {
+                const double r0 = hr0[(size_t)d * R + lc];
+                if (r0 > 0.0) worst = std::fmax(worst, std::sqrt(hrr[(size_t)d * R + lc] / r0));
+            }
+            std::fprintf(stderr, " %d:%lld/%.1e", d, (long long)hi[d], worst);
+        }
+        std::fprintf(stderr, "\n");
+    }
     bool any = false, changed = false;
     for (int d = 0; d < D; ++d) {
         bool stop = !h[d] && hi[d] >= cap;
@@ -2273,12 +2721,46 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
         float* r32 = b.r32.as<float>();
         float* p32 = b.p32.as<float>();
         float* z32 = mg ? b.z32.as<float>() : nullptr;
+        // with the multigrid, the 24 load cases of a domain iterate as one block
+        // (DTC_BLOCK_PCG=0: one PCG per load case)
+        const char* be = std::getenv("DTC_BLOCK_PCG");
+        const bool block = mg && !(be && be[0] == '0');
+        // the block's A P in FP32 (DTC_BLOCK_Q64=1: FP64 arithmetic)
+        const char* bq64 = std::getenv("DTC_BLOCK_Q64");
+        b.block_q64 = bq64 && bq64[0] == '1';
+        const char* brs = std::getenv("DTC_BLOCK_RESTART");
+        const bool block_restart = !(brs && brs[0] == '0');
+        // a block phase ends at its targets; the cap only bounds FP32 drift
+        const char* brp = std::getenv("DTC_BLOCK_REPLACE");
+        const int64_t block_replace = brp ? std::max(1, std::atoi(brp)) : 400;
+        float* bp = p32;
+        float* bq = b.q32.as<float>();
+        float* bz = z32;
+        if (block) DTC_TRY(block_setup(b));
         bool first = true;
         for (int it = 0; it < 100000; ++it) {
             bool left = false;
             DTC_TRY(apply_cap(b, cap, nouter, &left, &progress));
             if (!left) break;
-            k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.r64.as<double>(), r32, x32);
+            // the block solves the correction problem with unit-norm columns: FP32
+            // directions mix all 24 load cases, whose residuals drift apart in size
+            k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.r64.as<double>(), r32, x32,
+                                                 block ? b.rr.as<double>() : nullptr);
+            if (block) {
+                // the direction block restarts from M r at every residual replacement
+                // (DTC_BLOCK_RESTART=0: it is kept, spanning the same space after the
+                // column scaling)
+                if (first || block_restart) {
+                    DTC_TRY(mg_vcycle(b, wk32, r32, bz, nullptr));
+                    DTC_TRY(block_orth(b, bz, bp));
+                }
+                first = false;
+                DTC_TRY(block_run(b, wk32, x32, r32, bp, bq, bz, cap, block_replace));
+                k_acc64<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), x32, b.x64.as<double>(), b.rr.as<double>());
+                DTC_HIP(hipGetLastError());
+                DTC_TRY(residual64(b, wk64, b.x64.as<double>(), 0, o));
+                continue;
+            }
             if (mg) DTC_TRY(mg_vcycle(b, wk32, r32, z32, nullptr));
             k_cg_init<float><<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.active.as<int>(), x32, r32, p32,
                                                            b.dinv32.as<float>(), b.part_a.as<double>(),
@@ -2291,7 +2773,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
             DTC_TRY(pcg_run<float>(b, wk32, x32, r32, p32, b.q32.as<float>(), b.dinv32.as<float>(), cap, REPLACE,
                                    z32));
             DTC_HIP(hipMemcpyAsync(b.active_prev.p, b.active.p, (size_t)D * R * 4, hipMemcpyDeviceToDevice, s));
-            k_acc64<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), x32, b.x64.as<double>());
+            k_acc64<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), x32, b.x64.as<double>(), nullptr);
             DTC_HIP(hipGetLastError());
             DTC_TRY(residual64(b, wk64, b.x64.as<double>(), 0, o));
         }

@@ -75,6 +75,15 @@ struct SolverBatch {
     // multigrid preconditioner of the FP32 PCG (empty when Jacobi is used)
     std::vector<Level> mg;
     DevMem z32, t32, d32;          // fine-level V-cycle output, matvec output, Chebyshev direction
+    // block PCG of the mixed multigrid solve (the 24 load cases of a domain share one Krylov block)
+    int n_slices = 0;
+    DevMem slices;                 // int4 [slice]: domain, first node, end node
+    DevMem dslice;                 // int [dom + 1]: first slice of each domain
+    DevMem gpart;                  // double [slice][2][24 x 24]: Gram partial sums
+    DevMem gsum;                   // double [dom][2][24 x 24]: Gram sums
+    DevMem lfac;                   // double [dom][24 x 24]: Cholesky factor of P^T A P
+    DevMem bcoef;                  // float [dom][24 x 24]: alpha, then beta, of the current step
+    bool block_q64 = false;        // Q = A P in FP64 arithmetic
     int precond = 0;               // DTC_PC_JACOBI / DTC_PC_MG
     int mg_degree = 2;             // Chebyshev degree of the fine and intermediate levels
     double cheb_lmax = 0.0;        // element-by-element bound of lambda_max(D^-1 K)
