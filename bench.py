@@ -196,7 +196,7 @@ def main():
     fp64_rate = len(C64) / t64 if C64 else float("nan")
     terr = max((np.abs(res[i]["C"] - C64[i]).max() / np.abs(C64[i]).max() for i in range(len(C64))), default=float("nan"))
 
-    # ---- roofline of the dominant kernel (EbE matvec of the FP32 PCG)
+    # ---- roofline of the dominant kernel (EbE matvec Q = A P of the FP32 block PCG)
     mv_s = st["mv_avg_ms"] * 1e-3
     achieved = st["mv_bytes"] / mv_s / 1e9 if mv_s > 0 else 0.0
     traffic = None
@@ -207,7 +207,7 @@ def main():
             traffic = t["traffic_bytes_per_dispatch"]
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_matvec<float, IN_V, OUT_MASKED_DOT>", "avg_launch_ms": round(st["mv_avg_ms"], 5),
+            "kernel": "k_matvec<float, IN_V, OUT_MASKED>", "avg_launch_ms": round(st["mv_avg_ms"], 5),
             "launches_timed": st["mv_launches"], "algorithmic_bytes": st["mv_bytes"],
             "dense_equiv_tflops": round(st["mv_flops"] / mv_s / 1e12, 2) if mv_s > 0 else 0.0}
 

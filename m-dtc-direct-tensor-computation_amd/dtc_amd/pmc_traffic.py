@@ -10,8 +10,9 @@ Runs (as child processes, the program directly after `--`):
 FETCH_SIZE / WRITE_SIZE are reported in KiB per dispatch.  The copy reads and
 writes exactly 1 GiB, so `factor = 1 GiB / counter(copy)` converts each counter
 into bytes; traffic = FETCH * f_fetch + WRITE * f_write per dispatch of
-k_matvec<float, IN_V, OUT_MASKED_DOT>, averaged over its dispatches (the same
-launches whose HIP-event average and algorithmic bytes bench.py reports).
+k_matvec<float, IN_V, OUT_MASKED> (Q = A P of the block PCG), averaged over its
+dispatches (the same launches whose HIP-event average and algorithmic bytes
+bench.py reports).
 bench.py reports the result as roofline.traffic when its batch and domain edge
 match.
 """
@@ -27,7 +28,7 @@ import sys
 import tempfile
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-KERNEL = "k_matvec<float, 0, 0>"
+KERNEL = "k_matvec<float, 0, 3, false>"
 COPY = ("import torch; a = torch.ones(2**28, device='cuda'); b = torch.empty_like(a); torch.cuda.synchronize(); "
         "b.copy_(a); torch.cuda.synchronize(); print(float(b[-1]))")
 GIB = float(1 << 30)
@@ -89,7 +90,7 @@ def main() -> None:
     fetch_kib, n_f = res["FETCH_SIZE"]
     write_kib, n_w = res["WRITE_SIZE"]
     out = {
-        "kernel": "k_matvec<float, IN_V, OUT_MASKED_DOT>",
+        "kernel": "k_matvec<float, IN_V, OUT_MASKED>",
         "workload": f"bench.py batch {args.batch}, {args.n}^3 trabecular domains, one step of the mixed solve",
         "batch": args.batch, "domain_edge": args.n,
         "fetch_kib_per_dispatch": round(fetch_kib, 1), "write_kib_per_dispatch": round(write_kib, 1),
