@@ -887,12 +887,14 @@ constexpr double ORTH_DROP = 1e-10;   // of P^T P
 // Gram partial sums over the rows of one slice of a domain's nodes:
 //   g[0] = A^T B, g[1] = A^T C (NGR = 2; zero for NGR = 1), FP64 sums of FP32
 // products.  Lane (i, j) of a wave keeps rows 3i..3i+2 x columns 3j..3j+2; the
-// four waves take every fourth vector row and are summed in a fixed order.
+// four waves take every fourth vector row.  The loads of the next GU rows of a
+// wave are issued before the products of the current ones (a slice is a long
+// sequential walk per wave); the waves' sums are added in wave order.
 template <int NGR>
 __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices, const int* __restrict__ done,
                                                const float* __restrict__ A, const float* __restrict__ B,
                                                const float* __restrict__ C, double* __restrict__ gpart) {
-    __shared__ double s_g[4][2 * RR];
+    __shared__ double s_g[NGR * RR];
     const int4 sl = slices[blockIdx.x];   // domain, first node, end node
     if (done[sl.x]) return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -902,12 +904,10 @@ __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices,
     for (int t = 0; t < NGR; ++t)
 #pragma unroll
         for (int k = 0; k < 9; ++k) g[t][k] = 0.0;
-    // GU rows per wave and step, their loads issued together (a slice is a long
-    // sequential walk: one load round trip per step)
-    constexpr int GU = 2;
+    constexpr int GU = 4;
     const int64_t r1 = (int64_t)sl.z * 3;
-    for (int64_t row0 = (int64_t)sl.y * 3 + wave; row0 < r1; row0 += 4 * GU) {
-        float av[GU][3], bv[NGR][GU][3];
+    float av[GU][3], bv[NGR][GU][3];
+    auto load = [&](int64_t row0) {
 #pragma unroll
         for (int u = 0; u < GU; ++u) {
             const int64_t row = row0 + 4 * u;
@@ -919,33 +919,44 @@ __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices,
                 if (NGR == 2) bv[NGR - 1][u][i] = in ? C[row * R + cj + i] : 0.f;
             }
         }
+    };
+    int64_t row0 = (int64_t)sl.y * 3 + wave;
+    if (row0 < r1) load(row0);
+    for (; row0 < r1; row0 += 4 * GU) {
+        double ad[GU][3], bd[NGR][GU][3];
 #pragma unroll
-        for (int u = 0; u < GU; ++u) {
-            double ad[3], bd[NGR][3];
+        for (int u = 0; u < GU; ++u)
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                ad[i] = (double)av[u][i];
+                ad[u][i] = (double)av[u][i];
 #pragma unroll
-                for (int t = 0; t < NGR; ++t) bd[t][i] = (double)bv[t][u][i];
+                for (int t = 0; t < NGR; ++t) bd[t][u][i] = (double)bv[t][u][i];
             }
+        if (row0 + 4 * GU < r1) load(row0 + 4 * GU);
+#pragma unroll
+        for (int u = 0; u < GU; ++u)
 #pragma unroll
             for (int t = 0; t < NGR; ++t)
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) g[t][3 * i + j] = fma(ad[i], bd[t][j], g[t][3 * i + j]);
-        }
+                    for (int j = 0; j < 3; ++j) g[t][3 * i + j] = fma(ad[u][i], bd[t][u][j], g[t][3 * i + j]);
     }
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w)
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+            for (int t = 0; t < NGR; ++t)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            s_g[wave][(ci + i) * R + cj + j] = g[0][3 * i + j];
-            s_g[wave][RR + (ci + i) * R + cj + j] = NGR == 2 ? g[NGR - 1][3 * i + j] : 0.0;
-        }
-    __syncthreads();
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const int e = t * RR + (ci + i) * R + cj + j;
+                        s_g[e] = w == 0 ? g[t][3 * i + j] : s_g[e] + g[t][3 * i + j];
+                    }
+        __syncthreads();
+    }
     double* out = gpart + (int64_t)blockIdx.x * 2 * RR;
-    for (int e = tid; e < 2 * RR; e += GR_NT) out[e] = ((s_g[0][e] + s_g[1][e]) + s_g[2][e]) + s_g[3][e];
+    for (int e = tid; e < 2 * RR; e += GR_NT) out[e] = e < NGR * RR ? s_g[e] : 0.0;
 }
 
 // gsum[d] = sum over the domain's slices of the Gram partials, in a fixed order:
