@@ -906,41 +906,61 @@ __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices,
         for (int k = 0; k < 9; ++k) g[t][k] = 0.0;
     constexpr int GU = 4;
     const int64_t r1 = (int64_t)sl.z * 3;
-    float av[GU][3], bv[NGR][GU][3];
-    auto load = [&](int64_t row0) {
+    // two register sets: the next GU rows load into one while the products of the
+    // other run; full steps (all GU rows in the slice) load without guards
+    float av[2][GU][3], bv[2][NGR][GU][3];
+    auto load = [&](int set, int64_t row0) {
 #pragma unroll
         for (int u = 0; u < GU; ++u) {
             const int64_t row = row0 + 4 * u;
-            const bool in = row < r1;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                av[u][i] = in ? A[row * R + ci + i] : 0.f;
-                bv[0][u][i] = in ? B[row * R + cj + i] : 0.f;
-                if (NGR == 2) bv[NGR - 1][u][i] = in ? C[row * R + cj + i] : 0.f;
+                av[set][u][i] = A[row * R + ci + i];
+                bv[set][0][u][i] = B[row * R + cj + i];
+                if (NGR == 2) bv[set][NGR - 1][u][i] = C[row * R + cj + i];
             }
         }
     };
+    auto product = [&](const float (&a)[3], const float (&b)[3], double (&gg)[9]) {
+        double ad[3], bd[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            ad[i] = (double)a[i];
+            bd[i] = (double)b[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) gg[3 * i + j] = fma(ad[i], bd[j], gg[3 * i + j]);
+    };
+    auto products = [&](int set) {
+#pragma unroll
+        for (int u = 0; u < GU; ++u)
+#pragma unroll
+            for (int t = 0; t < NGR; ++t) product(av[set][u], bv[set][t][u], g[t]);
+    };
     int64_t row0 = (int64_t)sl.y * 3 + wave;
-    if (row0 < r1) load(row0);
-    for (; row0 < r1; row0 += 4 * GU) {
-        double ad[GU][3], bd[NGR][GU][3];
+    const int64_t full_end = r1 - 4 * (GU - 1);   // row0 < full_end: rows row0 + 4u < r1
+    if (row0 < full_end) load(0, row0);
+    while (row0 < full_end) {
+        if (row0 + 4 * GU < full_end) load(1, row0 + 4 * GU);
+        products(0);
+        row0 += 4 * GU;
+        if (row0 >= full_end) break;
+        if (row0 + 4 * GU < full_end) load(0, row0 + 4 * GU);
+        products(1);
+        row0 += 4 * GU;
+    }
+    for (; row0 < r1; row0 += 4) {      // the last rows of the wave (fewer than GU)
+        float a[3], b[NGR][3];
 #pragma unroll
-        for (int u = 0; u < GU; ++u)
+        for (int i = 0; i < 3; ++i) {
+            a[i] = A[row0 * R + ci + i];
+            b[0][i] = B[row0 * R + cj + i];
+            if (NGR == 2) b[NGR - 1][i] = C[row0 * R + cj + i];
+        }
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                ad[u][i] = (double)av[u][i];
-#pragma unroll
-                for (int t = 0; t < NGR; ++t) bd[t][u][i] = (double)bv[t][u][i];
-            }
-        if (row0 + 4 * GU < r1) load(row0 + 4 * GU);
-#pragma unroll
-        for (int u = 0; u < GU; ++u)
-#pragma unroll
-            for (int t = 0; t < NGR; ++t)
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) g[t][3 * i + j] = fma(ad[u][i], bd[t][u][j], g[t][3 * i + j]);
+        for (int t = 0; t < NGR; ++t) product(a, b[t], g[t]);
     }
     for (int w = 0; w < 4; ++w) {
         if (wave == w)
