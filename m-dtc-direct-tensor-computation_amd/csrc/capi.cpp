@@ -14,6 +14,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "bc.hpp"
@@ -262,6 +263,7 @@ struct dtc_batch {
     // nodes stay below NODE_CAP (32-bit vector indexing); each sub-batch keeps
     // its topology and solutions for the sensitivity pass.
     struct Sub {
+        Stream own;                // the sub-batch's stream
         SolverBatch sb;
         std::vector<double> ff;
         std::vector<int64_t> iters, outer;
@@ -488,6 +490,7 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
     b->g0.assign(D, 0);
     b->ng.assign(D, 0);
     std::vector<int> first(1, 0);
+    bool concurrent = false;
     {
         int64_t cap = NODE_CAP;   // DTC_SUBBATCH_NODES lowers it (smaller sub-batches)
         if (const char* e = std::getenv("DTC_SUBBATCH_NODES")) cap = std::max<int64_t>(1, std::min<int64_t>(cap, std::atoll(e)));
@@ -504,13 +507,45 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
             acc += est;
         }
         first.push_back(D);
+        // DTC_CONCURRENT=k (k > 1): a batch that fits one sub-batch is split into k
+        // parts of about equal size whose solves run at the same time on their own
+        // streams: the iterations of one part fill the gaps of the other (launch
+        // and reduction latency, and the tail where few domains are still active).
+        // Off by default: kernels of the parts overlap, so per-kernel timings
+        // (dtc_batch_stats) no longer measure one kernel alone.
+        int conc = 1;
+        if (const char* e = std::getenv("DTC_CONCURRENT")) conc = std::max(1, std::atoi(e));
+        conc = std::min(conc, D);
+        if (first.size() == 2 && conc > 1) {
+            int64_t total = 0;
+            std::vector<int64_t> est(D);
+            for (int i = 0; i < D; ++i) {
+                const dtc_domain_desc& d = b->d[i];
+                const int64_t m = d.micro_nen == 20 ? 2 : 1;
+                est[i] = (m * d.n[0] + 1) * (m * d.n[1] + 1) * (m * d.n[2] + 1) * b->ng[i];
+                total += est[i];
+            }
+            first.assign(1, 0);
+            int64_t acc = 0;
+            for (int i = 0; i < D; ++i) {
+                if (acc * conc >= total * (int64_t)first.size() && (int)first.size() < conc) first.push_back(i);
+                acc += est[i];
+            }
+            first.push_back(D);
+            concurrent = first.size() > 2;
+        }
     }
     const int NS = (int)first.size() - 1;
-    // one sub-batch: mesh -> topology -> solve -> reactions for the listed domains
-    auto run_sub = [&](int k, const std::vector<int>& list, int precision) -> int {
+    int device = 0;
+    DTC_HIP(hipGetDevice(&device));
+    // one sub-batch, set-up: mesh -> topology for the listed domains (sequential:
+    // the sub-batches share the mesh work space)
+    auto setup_sub = [&](int k, const std::vector<int>& list, int precision) -> int {
         while ((int)b->subs.size() <= k) b->subs.emplace_back(new dtc_batch::Sub);
         dtc_batch::Sub& S = *b->subs[k];
+        if (!S.own.s) DTC_TRY(S.own.create());
         SolverBatch& sb = S.sb;
+        const hipStream_t s = S.own.s;
         sb.stream = s;
         sb.nu = b->nu;
         sb.a = b->a0;
@@ -543,15 +578,47 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         sb.precond = (precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG && !sb.micro20) ? DTC_PC_MG
                                                                                                : DTC_PC_JACOBI;
         sb.mg_degree = b->o.mg_degree > 0 ? b->o.mg_degree : 2;
+        return DTC_OK;
+    };
+    // one sub-batch, solve: multigrid hierarchy -> solve -> reactions
+    auto solve_sub = [&](int k) -> int {
+        dtc_batch::Sub& S = *b->subs[k];
+        SolverBatch& sb = S.sb;
         if (sb.precond == DTC_PC_MG) DTC_TRY(batch_build_mg(sb, 8));
         DTC_TRY(batch_solve(sb, b->o, S.iters, S.outer, S.relres));
         DTC_TRY(batch_forces(sb, S.ff));
         return DTC_OK;
     };
+    auto run_sub = [&](int k, const std::vector<int>& list, int precision) -> int {
+        DTC_TRY(setup_sub(k, list, precision));
+        return solve_sub(k);
+    };
     for (int k = 0; k < NS; ++k) {
         std::vector<int> list;
         for (int i = first[k]; i < first[k + 1]; ++i) list.push_back(i);
-        DTC_TRY(run_sub(k, list, b->o.precision));
+        DTC_TRY(setup_sub(k, list, b->o.precision));
+    }
+    if (!concurrent) {
+        for (int k = 0; k < NS; ++k) DTC_TRY(solve_sub(k));
+    } else {
+        // concurrent solves (one host thread per sub-batch, each on its stream);
+        // the error message of a failed solve is carried back to this thread
+        std::vector<int> rc(NS, DTC_OK);
+        std::vector<std::string> msg(NS);
+        std::vector<std::thread> th;
+        for (int k = 0; k < NS; ++k)
+            th.emplace_back([&, k] {
+                if (hipSetDevice(device) != hipSuccess) {
+                    rc[k] = DTC_ERR_DEVICE;
+                    msg[k] = "hipSetDevice failed in a solve thread";
+                    return;
+                }
+                rc[k] = solve_sub(k);
+                if (rc[k] != DTC_OK) msg[k] = dtc_last_error();
+            });
+        for (std::thread& t : th) t.join();
+        for (int k = 0; k < NS; ++k)
+            if (rc[k] != DTC_OK) return set_error(rc[k], "%s", msg[k].c_str());
     }
     // A mixed-precision solve that did not converge (a near-singular mode, e.g. parts
     // hinged on one voxel edge, can grow without bound in the FP32 inner solve) is

@@ -1849,7 +1849,7 @@ int walsh_blocks(const double* Khat, WalshK* out) {
         return s;
     };
     // Kw[(m,d),(m',d')] = sum H[m][b] Khat[(l(b),d),(l(b'),d')] H[m'][b'] / 64
-    static double Kw[24][24];
+    double Kw[24][24];
     double kmax = 0.0;
     for (int m = 0; m < 8; ++m)
         for (int d = 0; d < 3; ++d)

@@ -316,6 +316,31 @@ def test_block_pcg_matches_one_pcg_per_load_case(monkeypatch):
     assert sum(b["iterations"] for b in rb) < 0.7 * sum(a["iterations"] for a in rs)
 
 
+def test_concurrent_sub_batches_match_sequential_solves(monkeypatch):
+    """DTC_CONCURRENT=2: a batch that fits one sub-batch is split into two halves
+    whose solves run at the same time on their own streams (one host thread each);
+    the results, the error statuses and the sensitivity equal those of the batch
+    solved as one."""
+    dtc = _dtc()
+    from dtc_amd import synth
+    doms = [dtc.Domain(synth.trabecular(24, seed=30 + i, sigma=1.5, bvtv=0.3), delta=(0.01,) * 3) for i in range(5)]
+    doms.append(dtc.Domain(np.zeros((8, 8, 8), np.int8), delta=(0.01,) * 3))      # empty: DTC_ERR_EMPTY_MESH
+    kw = dict(precision=1, rel_tol=1e-8, rotation=True, precond=dtc.DTC_PC_MG)
+    dL = np.eye(6)
+    monkeypatch.setenv("DTC_CONCURRENT", "2")
+    with dtc.Batch(doms, **kw) as b:
+        rc = b.run()
+        gc = b.grad(2, dL)
+    monkeypatch.setenv("DTC_CONCURRENT", "1")
+    with dtc.Batch(doms, **kw) as b:
+        rs = b.run()
+        gs = b.grad(2, dL)
+    assert [r["status"] for r in rc] == [r["status"] for r in rs] == [0] * 5 + [dtc.DTC_ERR_EMPTY_MESH]
+    for a, c in zip(rc[:5], rs[:5]):
+        assert _rel(a["C"], c["C"]) < 1e-9 and _rel(a["K_macro"], c["K_macro"]) < 1e-9
+    assert np.abs(gc - gs).max() <= 1e-8 * np.abs(gs).max()
+
+
 def test_block_pcg_reaches_tight_tolerance_on_greyscale():
     """Greyscale moduli at rel-res 1e-11: the residual columns of the later FP64
     replacements line up with the slowest modes, so the block drops dependent
