@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg",
                     help="preconditioner of the batched mixed-precision PCG")
     ap.add_argument("--mg-degree", type=int, default=0, help="Chebyshev degree per smoothing pass (0 = library default)")
+    ap.add_argument("--inner-tol", type=float, default=0.0,
+                    help="FP32 inner reduction before an FP64 residual replacement (0 = library default)")
     ap.add_argument("--uneven", type=int, default=0,
                     help="instead: one batch of this many uneven domains (edges 32..96, BV/TV 0.08..0.35, "
                          "anisotropic frames; rotation search on a 5 degree grid)")
@@ -147,7 +149,7 @@ def main():
 
     # ---- batched FP32 + FP64 refinement
     pc = dtc_amd.DTC_PC_MG if args.precond == "mg" else dtc_amd.DTC_PC_JACOBI
-    b = dtc_amd.Batch(doms, precision=dtc_amd.DTC_FP32_REFINE, rel_tol=1e-8, inner_tol=1e-4, rotation=True,
+    b = dtc_amd.Batch(doms, precision=dtc_amd.DTC_FP32_REFINE, rel_tol=1e-8, inner_tol=args.inner_tol, rotation=True,
                       max_iter=args.max_iter, precond=pc, mg_degree=args.mg_degree)
     for w in range(args.warmup):
         res = b.run()

@@ -110,7 +110,9 @@ typedef struct {
     int precision;          /* DTC_FP64 or DTC_FP32_REFINE */
     int rotation;           /* 0: skip; 1: CR_1 and CR_2 searches */
     double rel_tol;         /* true ||f - K u||_2 / ||f||_2 per load case (e.g. 1e-8) */
-    double inner_tol;       /* FP32_REFINE: relative reduction per inner solve (e.g. 1e-4) */
+    double inner_tol;       /* FP32_REFINE: relative reduction per inner solve; 0 = automatic
+                               (1e-6 for the multigrid block PCG when the per-voxel moduli
+                               differ by at most 100x, 1e-4 otherwise) */
     int64_t max_iter;       /* cap on the total PCG iterations */
     double rot_step_deg;    /* initial grid step of the rotation search (1 = reference) */
     double num_zero;        /* refinement stop |dmin| < num_zero (1e-9 in mod_tensor_opt.f90:61) */

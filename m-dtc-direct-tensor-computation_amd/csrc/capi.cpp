@@ -259,6 +259,7 @@ struct dtc_batch {
     Stream st;
     std::vector<std::unique_ptr<DevMem>> phi, mod;
     std::vector<double> escale;
+    double contrast = 1.0;         // largest per-voxel modulus contrast of the batch
     // Domains are solved in sub-batches of consecutive domains whose solver
     // nodes stay below NODE_CAP (32-bit vector indexing); each sub-batch keeps
     // its topology and solutions for the sensitivity pass.
@@ -446,7 +447,7 @@ int dtc_batch_create(const dtc_domain_desc* d, int64_t n_dom, const dtc_solve_op
     std::unique_ptr<dtc_batch> b(new dtc_batch);
     b->d.assign(d, d + n_dom);
     b->o = *o;
-    if (!(b->o.inner_tol > 0.0 && b->o.inner_tol < 1.0)) b->o.inner_tol = 1e-4;
+    if (!(b->o.inner_tol > 0.0 && b->o.inner_tol < 1.0)) b->o.inner_tol = 0.0;   // automatic (batch_solve)
     if (!(b->o.rot_step_deg > 0.0)) b->o.rot_step_deg = 1.0;
     if (!(b->o.num_zero > 0.0)) b->o.num_zero = 1e-9;
     DTC_TRY(b->st.create());
@@ -459,6 +460,17 @@ int dtc_batch_create(const dtc_domain_desc* d, int64_t n_dom, const dtc_solve_op
         if (d[i].modulus) {
             DTC_TRY(m->alloc(nv * 8));
             DTC_HIP(hipMemcpyAsync(m->p, d[i].modulus, nv * 8, hipMemcpyHostToDevice, b->st.s));
+            // modulus contrast (max / min positive), which bounds how deep an FP32
+            // inner solve can go (the automatic inner_tol)
+            double lo = INFINITY, hi = 0.0;
+            for (size_t v = 0; v < nv; ++v) {
+                const double e = d[i].modulus[v];
+                if (e > 0.0) {
+                    lo = std::min(lo, e);
+                    hi = std::max(hi, e);
+                }
+            }
+            if (hi > 0.0) b->contrast = std::max(b->contrast, hi / lo);
         }
         b->phi.push_back(std::move(p));
         b->mod.push_back(std::move(m));
@@ -547,6 +559,7 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         SolverBatch& sb = S.sb;
         const hipStream_t s = S.own.s;
         sb.stream = s;
+        sb.contrast = b->contrast;
         sb.nu = b->nu;
         sb.a = b->a0;
         sb.micro20 = b->d[0].micro_nen == 20;

@@ -2689,9 +2689,22 @@ int apply_cap(SolverBatch& b, int64_t cap, std::vector<int64_t>& nouter, bool* a
 
 }  // namespace
 
-int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& iters, std::vector<int64_t>& outer,
-                std::vector<double>& relres) {
+int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>& iters,
+                std::vector<int64_t>& outer, std::vector<double>& relres) {
     const int D = b.n_dom;
+    // with the multigrid, the 24 load cases of a domain iterate as one block
+    // (DTC_BLOCK_PCG=0: one PCG per load case)
+    const char* be = std::getenv("DTC_BLOCK_PCG");
+    const bool block = b.precision == DTC_FP32_REFINE && b.precond == DTC_PC_MG && !b.micro20 &&
+                       !(be && be[0] == '0');
+    // inner_tol outside (0, 1): automatic.  The block restarts at every FP64
+    // residual replacement, so it takes its FP32 phases as deep as FP32 allows:
+    // 1e-6 (two phases to 1e-8 instead of three) where the voxel moduli differ by
+    // at most 100x; high-contrast (greyscale) domains stall in FP32 before that
+    // and use 1e-4, as does one PCG per load case (it keeps its direction over a
+    // replacement)
+    dtc_solve_opts o = opts;
+    if (!(o.inner_tol > 0.0 && o.inner_tol < 1.0)) o.inner_tol = block && b.contrast <= 100.0 ? 1e-6 : 1e-4;
     iters.assign(D, 0);
     outer.assign(D, 0);
     relres.assign(D, 0.0);
@@ -2752,10 +2765,6 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& i
         float* r32 = b.r32.as<float>();
         float* p32 = b.p32.as<float>();
         float* z32 = mg ? b.z32.as<float>() : nullptr;
-        // with the multigrid, the 24 load cases of a domain iterate as one block
-        // (DTC_BLOCK_PCG=0: one PCG per load case)
-        const char* be = std::getenv("DTC_BLOCK_PCG");
-        const bool block = mg && !(be && be[0] == '0');
         // the block's A P in FP32 (DTC_BLOCK_Q64=1: FP64 arithmetic)
         const char* bq64 = std::getenv("DTC_BLOCK_Q64");
         b.block_q64 = bq64 && bq64[0] == '1';

@@ -96,6 +96,7 @@ struct SolverBatch {
     WalshK wk64{};
     double kdiag = 0.0;            // diagonal entry of the E = 1 element stiffness
     double nu = 0.3, a = 1.0;
+    double contrast = 1.0;         // per-voxel modulus contrast (max / min) of the batch
     int precision = DTC_FP64;
     bool reactions_valid = false;
     bool capturing = false;

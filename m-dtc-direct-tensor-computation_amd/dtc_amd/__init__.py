@@ -183,7 +183,7 @@ class Domain:
         return d
 
 
-def opts(precision=DTC_FP32_REFINE, rel_tol=1e-8, inner_tol=1e-4, max_iter=200000, rotation=True, rot_step_deg=1.0,
+def opts(precision=DTC_FP32_REFINE, rel_tol=1e-8, inner_tol=0.0, max_iter=200000, rotation=True, rot_step_deg=1.0,
          num_zero=1e-9, precond=DTC_PC_JACOBI, mg_degree=0) -> SolveOpts:
     o = SolveOpts()
     o.precision, o.rotation, o.rel_tol, o.inner_tol = int(precision), int(bool(rotation)), float(rel_tol), float(inner_tol)

@@ -45,7 +45,7 @@ module dtc_gpu
      integer(c_int)       :: precision = DTC_FP32_REFINE
      integer(c_int)       :: rotation = 1
      real(c_double)       :: rel_tol = 1.e-8_c_double
-     real(c_double)       :: inner_tol = 1.e-4_c_double
+     real(c_double)       :: inner_tol = 0._c_double    ! 0: automatic
      integer(c_int64_t)   :: max_iter = 200000
      real(c_double)       :: rot_step_deg = 1._c_double
      real(c_double)       :: num_zero = 1.e-9_c_double
