@@ -2333,7 +2333,7 @@ struct LvView {
 
 // Chebyshev smoothing parameters: interval [lmax / ratio, lmax] of D^-1 K
 struct MgParams {
-    double ratio = 5.0;        // smoothing passes of the fine and intermediate levels
+    double ratio = 7.0;        // smoothing passes of the fine and intermediate levels
     int coarse_degree = 4;     // coarsest level
     double coarse_ratio = 50.0;
     double lscale = 1.0;       // multiplies the element-by-element bound of lambda_max
