@@ -8,7 +8,9 @@ pass of the whole path over a batch of such domains resident in HBM: mesh ->
 boundary conditions -> 24 load-case solves to a true relative residual of
 1e-8 -> force projection and effective stiffness -> CR_1/CR_2 rotation search.
 
-value  = domains per second of the batched FP32 + FP64-refine solve.
+value  = domains per second of the batched FP32 + FP64-refine solve (block PCG:
+the 24 load cases of a domain share one Krylov block, multigrid V-cycle
+preconditioner).
 Also reported: the plain FP64 GPU form (one domain per launch sequence), the
 tensor error of the mixed solve against it, the EbE matvec roofline and the
 CPU FP64 restatement (the oracle) timed on a bounded sample on this host.
