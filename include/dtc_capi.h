@@ -122,7 +122,8 @@ typedef struct {
 
 typedef struct {
     int status;                 /* DTC_OK or DTC_ERR_* for this domain */
-    int _pad;
+    int precision_used;         /* DTC_FP32_REFINE, or DTC_FP64 when the solve ran in FP64 (requested,
+                                   or the mixed solve stalled and the domain was repeated in FP64) */
     int64_t n_elems, n_nodes, n_bnodes;
     int64_t iterations;         /* PCG iterations (inner, summed) */
     int64_t outer_iterations;   /* FP64 residual evaluations */
@@ -172,6 +173,23 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
  * domains are skipped by the kernels and not counted). */
 int dtc_batch_stats(dtc_batch* b, double* mv_avg_ms, int64_t* mv_launches, double* mv_bytes,
                     double* mv_flops, double* run_ms);
+/* kernel timing of the following runs on (1) or off (0, the default): HIP
+ * events on the solve stream around the fine-level launches of the timed
+ * kernels below.  No reference counterpart (the reference's timers,
+ * f-src/mod_times.f90:25,99, time whole program sections). */
+int dtc_batch_timing(dtc_batch* b, int enable);
+#define DTC_KSTAT_MV    0   /* CG matvec: Q = K P of the block PCG / q = K p of the PCG (FP32) */
+#define DTC_KSTAT_CHEB  1   /* Chebyshev-Jacobi smoothing step fused into the matvec (fine level) */
+#define DTC_KSTAT_BRES  2   /* b - K x before restriction (fine level) */
+#define DTC_KSTAT_RESID 3   /* FP64 true residual f - K u */
+#define DTC_KSTAT_COUNT 4
+/* average duration (ms) and average algorithmic bytes of one launch of timed
+ * kernel `kernel` (DTC_KSTAT_*) over the last timed run, and the number of
+ * launches.  Bytes count each vector the launch reads or writes once (3 N_n R s
+ * per vector), the element moduli (4 N_e) and the Jacobi scaling (12 N_n) of
+ * the domains still iterating in that launch.  The FP64 repeat of domains whose
+ * mixed solve stalled is not included. */
+int dtc_batch_kernel_stats(dtc_batch* b, int kernel, double* avg_ms, int64_t* launches, double* avg_bytes);
 int dtc_batch_destroy(dtc_batch* b);
 
 /* ---- a12: rotation search ------------------------------------------------

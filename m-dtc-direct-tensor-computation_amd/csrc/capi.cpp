@@ -52,6 +52,12 @@ struct Stream {
 };
 
 int kind_size(int kind) { return kind == DTC_IK1 ? 1 : kind == DTC_IK2 ? 2 : kind == DTC_IK4 ? 4 : 0; }
+// signed voxel value v of a host block of kind ik1/ik2/ik4
+int64_t voxel_value(const void* phi, int kind, size_t v) {
+    return kind == DTC_IK1 ? (int64_t) static_cast<const int8_t*>(phi)[v]
+         : kind == DTC_IK2 ? (int64_t) static_cast<const int16_t*>(phi)[v]
+                           : (int64_t) static_cast<const int32_t*>(phi)[v];
+}
 
 int check_device() {
     int n = 0;
@@ -259,7 +265,7 @@ struct dtc_batch {
     Stream st;
     std::vector<std::unique_ptr<DevMem>> phi, mod;
     std::vector<double> escale;
-    double contrast = 1.0;         // largest per-voxel modulus contrast of the batch
+    std::vector<double> contrast;  // per domain: modulus contrast (max / min) over its solid voxels
     // Domains are solved in sub-batches of consecutive domains whose solver
     // nodes stay below NODE_CAP (32-bit vector indexing); each sub-batch keeps
     // its topology and solutions for the sensitivity pass.
@@ -272,6 +278,8 @@ struct dtc_batch {
     };
     std::vector<std::unique_ptr<Sub>> subs;
     int n_sub = 0;                 // sub-batches used by the last run
+    int n_main = 0;                // of which solved in the requested precision (the rest: FP64 repeats)
+    bool timing = false;           // kernel timing (dtc_batch_timing)
     std::vector<int> sbi, g0, ng;  // sub-batch, first solver group and group count of each domain
     double nu = 0.3, a0 = 1.0;
     std::vector<double> khat20;    // HEX20 micro: E = 1 element stiffness
@@ -455,23 +463,26 @@ int dtc_batch_create(const dtc_domain_desc* d, int64_t n_dom, const dtc_solve_op
     for (int64_t i = 0; i < n_dom; ++i) {
         const size_t nv = (size_t)d[i].n[0] * d[i].n[1] * d[i].n[2];
         std::unique_ptr<DevMem> p(new DevMem), m(new DevMem);
+        double ctr = 1.0;
         DTC_TRY(p->alloc(nv * kind_size(d[i].phi_kind)));
         DTC_HIP(hipMemcpyAsync(p->p, d[i].phi, nv * kind_size(d[i].phi_kind), hipMemcpyHostToDevice, b->st.s));
         if (d[i].modulus) {
             DTC_TRY(m->alloc(nv * 8));
             DTC_HIP(hipMemcpyAsync(m->p, d[i].modulus, nv * 8, hipMemcpyHostToDevice, b->st.s));
-            // modulus contrast (max / min positive), which bounds how deep an FP32
-            // inner solve can go (the automatic inner_tol)
+            // modulus contrast (max / min positive) over the solid voxels (phi >=
+            // llimit), which bounds how deep an FP32 inner solve can go (the
+            // automatic inner_tol of the domain)
             double lo = INFINITY, hi = 0.0;
             for (size_t v = 0; v < nv; ++v) {
                 const double e = d[i].modulus[v];
-                if (e > 0.0) {
+                if (e > 0.0 && voxel_value(d[i].phi, d[i].phi_kind, v) >= d[i].llimit) {
                     lo = std::min(lo, e);
                     hi = std::max(hi, e);
                 }
             }
-            if (hi > 0.0) b->contrast = std::max(b->contrast, hi / lo);
+            if (hi > 0.0) ctr = hi / lo;
         }
+        b->contrast.push_back(ctr);
         b->phi.push_back(std::move(p));
         b->mod.push_back(std::move(m));
         b->escale.push_back(d[i].size_domain * d[i].delta[0] / a0);
@@ -559,12 +570,11 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         SolverBatch& sb = S.sb;
         const hipStream_t s = S.own.s;
         sb.stream = s;
-        sb.contrast = b->contrast;
         sb.nu = b->nu;
         sb.a = b->a0;
         sb.micro20 = b->d[0].micro_nen == 20;
         sb.khat20 = b->khat20;
-        sb.timing = true;
+        sb.timing = b->timing;
         std::vector<dtc_domain_desc> gd;
         for (int i : list) {
             b->sbi[i] = k;
@@ -572,6 +582,8 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
             for (int g = 0; g < b->ng[i]; ++g) gd.push_back(b->d[i]);
         }
         DTC_TRY(batch_begin(sb, gd.data(), (int64_t)gd.size()));
+        for (int i : list)
+            for (int g = 0; g < b->ng[i]; ++g) sb.h_contrast.push_back(b->contrast[i]);
         for (int i : list) {
             const dtc_domain_desc& d = b->d[i];
             int rc = mesh_build_device(b->phi[i]->p, d.phi_kind, d.n, d.llimit, d.micro_nen, b->mesh, s);
@@ -637,6 +649,7 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
     // hinged on one voxel edge, can grow without bound in the FP32 inner solve) is
     // repeated for that domain in FP64 with Jacobi-PCG, the reference's own scheme.
     int nsub = NS;
+    b->n_main = NS;
     if (b->o.precision == DTC_FP32_REFINE) {
         std::vector<int> retry;
         for (int i = 0; i < D; ++i) {
@@ -662,6 +675,7 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         const dtc_batch::Sub& S = *b->subs[b->sbi[i]];
         const int gi = b->g0[i], RM = macro_lcs(d.macro_order), FD = 3 * macro_nodes(d.macro_order);
         r.status = S.sb.h_status[gi];
+        r.precision_used = S.sb.precision;
         for (int g = 0; g < b->ng[i]; ++g) {
             const int st = S.sb.h_status[gi + g];
             if (r.status == DTC_OK && st != DTC_OK) r.status = st;
@@ -755,30 +769,47 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
 int dtc_batch_stats(dtc_batch* b, double* mv_avg_ms, int64_t* mv_launches, double* mv_bytes, double* mv_flops,
                     double* run_ms) {
     if (!b) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL batch");
+    double avg = 0.0, bytes = 0.0;
     int64_t n = 0;
-    double ms = 0.0, bytes = 0.0, flops = 0.0;
-    // B_mv = 2 (3 N_n R s) + 4 N_e, F_mv = 1152 R N_e (dense-equivalent) per domain
-    // group, counted in every timed launch in which the group was still
-    // iterating (its PCG iteration count), averaged over the timed launches
-    for (int k = 0; k < b->n_sub && k < (int)b->subs.size(); ++k) {
+    DTC_TRY(dtc_batch_kernel_stats(b, DTC_KSTAT_MV, &avg, &n, &bytes));
+    // dense-equivalent flops F_mv = 1152 R N_e per domain group, counted in as
+    // many launches as the group took PCG iterations, averaged over the launches
+    double flops = 0.0;
+    for (int k = 0; k < b->n_main && k < (int)b->subs.size(); ++k) {
         const auto& S = b->subs[k];
-        const double s = S->sb.precision == DTC_FP64 ? 8.0 : 4.0;
-        n += S->sb.mv_count;
-        ms += S->sb.mv_ms_sum;
         for (size_t i = 0; i < S->sb.h_elems.size() && i < S->iters.size(); ++i)
-            if (S->sb.h_status[i] == DTC_OK || S->sb.h_status[i] == DTC_ERR_NO_CONVERGENCE) {
-                const double ne = (double)S->sb.h_elems[i], nn = (double)S->sb.h_nodes[i],
-                             it = (double)S->iters[i];
-                bytes += it * (2.0 * (3.0 * nn * R * s) + 4.0 * ne);
-                flops += it * 1152.0 * R * ne;
-            }
+            if (S->sb.h_status[i] == DTC_OK || S->sb.h_status[i] == DTC_ERR_NO_CONVERGENCE)
+                flops += (double)S->iters[i] * 1152.0 * R * (double)S->sb.h_elems[i];
     }
-    const double avg = n ? ms / (double)n : 0.0;
     if (mv_avg_ms) *mv_avg_ms = avg;
     if (mv_launches) *mv_launches = n;
-    if (mv_bytes) *mv_bytes = n ? bytes / (double)n : 0.0;
+    if (mv_bytes) *mv_bytes = bytes;
     if (mv_flops) *mv_flops = n ? flops / (double)n : 0.0;
     if (run_ms) *run_ms = b->run_ms;
+    return DTC_OK;
+}
+
+int dtc_batch_timing(dtc_batch* b, int enable) {
+    if (!b) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL batch");
+    b->timing = enable != 0;
+    return DTC_OK;
+}
+
+int dtc_batch_kernel_stats(dtc_batch* b, int kernel, double* avg_ms, int64_t* launches, double* avg_bytes) {
+    if (!b) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL batch");
+    if (kernel < 0 || kernel >= DTC_KSTAT_COUNT) return set_error(DTC_ERR_BAD_ARGUMENT, "kernel %d", kernel);
+    double ms = 0.0, bytes = 0.0;
+    int64_t n = 0;
+    // the sub-batches of the requested precision (FP64 repeats of stalled domains excluded)
+    for (int k = 0; k < b->n_main && k < (int)b->subs.size(); ++k) {
+        const KernelStat& q = b->subs[k]->sb.kstat[kernel];
+        ms += q.ms_sum;
+        bytes += q.bytes_sum;
+        n += q.count;
+    }
+    if (avg_ms) *avg_ms = n ? ms / (double)n : 0.0;
+    if (launches) *launches = n;
+    if (avg_bytes) *avg_bytes = n ? bytes / (double)n : 0.0;
     return DTC_OK;
 }
 

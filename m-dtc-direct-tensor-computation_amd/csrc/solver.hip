@@ -806,9 +806,10 @@ __global__ __launch_bounds__(RED_NT) void k_red_beta(const DomInfo* __restrict__
 // load cases above tol^2 ||f||^2 stay active.  The recursive PCG residual target
 // is the larger of (tol/2)^2 ||f||^2 (so the true residual check passes after
 // drift) and itol2 * rr (an inner solve reduces the residual it started from by
-// at most sqrt(itol2); itol2 = 0: no relative target)
+// at most sqrt(itol2), itol2 per domain; null: no relative target)
 __global__ __launch_bounds__(RED_NT) void k_red_outer(const DomInfo* __restrict__ doms, const double* __restrict__ prr,
-                                                      int first, double tol2, double itol2, double* __restrict__ rr0,
+                                                      int first, double tol2, const double* __restrict__ itol2v,
+                                                      double* __restrict__ rr0,
                                                       double* __restrict__ rrcur, int* __restrict__ active64,
                                                       int* __restrict__ active, double* __restrict__ thr2,
                                                       int* __restrict__ done64, int* __restrict__ done) {
@@ -819,6 +820,7 @@ __global__ __launch_bounds__(RED_NT) void k_red_outer(const DomInfo* __restrict_
     dom_sum24(prr, D.brick_begin, D.brick_end, s_out, rr);
     if (tid == 0) {
         int any = 0;
+        const double itol2 = itol2v ? itol2v[d] : 0.0;
         for (int lc = 0; lc < R; ++lc) {
             if (first) rr0[d * R + lc] = rr[lc];
             const double r0 = rr0[d * R + lc];
@@ -1194,6 +1196,28 @@ __global__ __launch_bounds__(RED_NT) void k_bcheck(const DomInfo* __restrict__ d
         iters[d] = it;
         if (!any || it >= max_iters) done[d] = 1;
     }
+}
+
+// algorithmic bytes of one timed launch: over the domains not done, vpass
+// vectors (FP32, or FP64 with fp64), the element moduli (4 or 8 bytes each) and,
+// with jac, the FP32 Jacobi scaling; dom_bytes[d] = {V32, E32, J32, V64}.
+// Summed by one workgroup in a fixed order.
+__global__ __launch_bounds__(256) void k_launch_bytes(const int* __restrict__ done, const double* __restrict__ dom_bytes,
+                                                      int n_dom, int vpass, int fp64, int jac, double* __restrict__ out) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int d = threadIdx.x; d < n_dom; d += 256) {
+        if (done && done[d]) continue;
+        const double* q = dom_bytes + 4 * d;
+        acc += vpass * (fp64 ? q[3] : q[0]) + (fp64 ? 2.0 : 1.0) * q[1] + (jac ? q[2] : 0.0);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
 }
 
 __global__ void k_fill_f64(double* __restrict__ a, int64_t n, double v) {
@@ -1901,6 +1925,8 @@ int batch_begin(SolverBatch& b, const dtc_domain_desc* d, int64_t n_dom) {
     b.n_dom = 0; b.n_nodes = 0; b.n_lat = 0; b.n_vox = 0;
     b.h_dom.clear(); b.h_brick.clear(); b.h_elems.clear(); b.h_nodes.clear(); b.h_bnodes.clear(); b.h_status.clear();
     b.d20.clear();
+    b.h_contrast.clear();
+    b.n_slices = 0;   // the block PCG slices follow the batch's domains (block_setup)
     return DTC_OK;
 }
 
@@ -2243,7 +2269,13 @@ int batch_build_mg(SolverBatch& b, int max_levels) {
 
 namespace {
 
-int event_pair(SolverBatch& b, hipEvent_t* e0, hipEvent_t* e1) {
+constexpr int MAX_SLOTS = 4096;   // timed launches between two stream synchronisations
+
+// Start of a timed launch of kernel kind `kind`: records the first event on the
+// batch stream.  *slot = -1 when timing is off (or the slot table is full).
+int timed_begin(SolverBatch& b, int kind, int* slot) {
+    *slot = -1;
+    if (!b.timing || b.n_slots >= MAX_SLOTS) return DTC_OK;
     if (b.ev_free.size() < 2) {
         for (int k = 0; k < 2; ++k) {
             hipEvent_t e;
@@ -2252,24 +2284,46 @@ int event_pair(SolverBatch& b, hipEvent_t* e0, hipEvent_t* e1) {
             b.ev_free.push_back(e);
         }
     }
-    *e1 = b.ev_free.back(); b.ev_free.pop_back();
-    *e0 = b.ev_free.back(); b.ev_free.pop_back();
-    b.ev_used.push_back(*e0);
-    b.ev_used.push_back(*e1);
+    SolverBatch::Timed t;
+    t.e1 = b.ev_free.back(); b.ev_free.pop_back();
+    t.e0 = b.ev_free.back(); b.ev_free.pop_back();
+    t.kind = kind;
+    t.slot = b.n_slots++;
+    DTC_HIP(hipEventRecord(t.e0, b.stream));
+    b.ev_used.push_back(t);
+    *slot = t.slot;
+    return DTC_OK;
+}
+
+// End of a timed launch: second event, then the launch's algorithmic bytes over
+// the domains still iterating: vpass vectors of the working precision, the
+// element moduli and (jac) the Jacobi scaling
+int timed_end(SolverBatch& b, int slot, int vpass, bool fp64, bool jac, const int* done) {
+    if (slot < 0) return DTC_OK;
+    DTC_HIP(hipEventRecord(b.ev_used.back().e1, b.stream));
+    k_launch_bytes<<<1, 256, 0, b.stream>>>(done, b.dom_bytes.as<double>(), b.n_dom, vpass, fp64 ? 1 : 0, jac ? 1 : 0,
+                                           b.launch_bytes.as<double>() + slot);
+    DTC_HIP(hipGetLastError());
     return DTC_OK;
 }
 
 // after a stream synchronisation: accumulate the timed launches
 int harvest_events(SolverBatch& b) {
-    for (size_t k = 0; k + 1 < b.ev_used.size(); k += 2) {
+    if (b.ev_used.empty()) return DTC_OK;
+    std::vector<double> by((size_t)b.n_slots);
+    DTC_HIP(hipMemcpy(by.data(), b.launch_bytes.p, by.size() * 8, hipMemcpyDeviceToHost));
+    for (const SolverBatch::Timed& t : b.ev_used) {
         float ms = 0.f;
-        DTC_HIP(hipEventElapsedTime(&ms, b.ev_used[k], b.ev_used[k + 1]));
-        b.mv_ms_sum += ms;
-        b.mv_count += 1;
-        b.ev_free.push_back(b.ev_used[k]);
-        b.ev_free.push_back(b.ev_used[k + 1]);
+        DTC_HIP(hipEventElapsedTime(&ms, t.e0, t.e1));
+        KernelStat& k = b.kstat[t.kind];
+        k.ms_sum += ms;
+        k.bytes_sum += by[t.slot];
+        k.count += 1;
+        b.ev_free.push_back(t.e0);
+        b.ev_free.push_back(t.e1);
     }
     b.ev_used.clear();
+    b.n_slots = 0;
     return DTC_OK;
 }
 
@@ -2278,14 +2332,12 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
     if (b.n_bricks == 0) return DTC_OK;
     MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(), b.einfo.as<int>(),
              sizeof(T) == 4 ? b.eE32.p : b.eE64.p, done, part};
-    // the CG matvec (OUT_MASKED: of the block PCG) is the dominant kernel: bracket it
-    // with events on the batch stream
-    const bool t = b.timing && IN == IN_V && (OUT == OUT_MASKED_DOT || OUT == OUT_MASKED);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (t) {
-        DTC_TRY(event_pair(b, &e0, &e1));
-        DTC_HIP(hipEventRecord(e0, b.stream));
-    }
+    // timed: the CG matvec (OUT_MASKED: of the block PCG) and the FP64 true residual
+    constexpr int kind = (IN == IN_V && (OUT == OUT_MASKED_DOT || OUT == OUT_MASKED)) ? DTC_KSTAT_MV
+                       : (IN == IN_U && OUT == OUT_RESID)                              ? DTC_KSTAT_RESID
+                                                                                       : -1;
+    int slot = -1;
+    if (kind >= 0) DTC_TRY(timed_begin(b, kind, &slot));
     if (b.micro20) {
         DTC_TRY(mv20(b, sizeof(T) == 8, IN, OUT, xin, yout, part, done));
     } else {
@@ -2293,7 +2345,8 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
         k_matvec<T, IN, OUT><<<grid, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
         DTC_HIP(hipGetLastError());
     }
-    if (t) DTC_HIP(hipEventRecord(e1, b.stream));
+    // IN_U without x (x = 0) reads no vector
+    DTC_TRY(timed_end(b, slot, (IN == IN_U && !xin) ? 1 : 2, sizeof(T) == 8, false, done));
     return DTC_OK;
 }
 
@@ -2308,15 +2361,26 @@ struct MvTopo {
     const void* eE32;
 };
 
+// tb: the batch whose fine level this is (its launches are timed), else null
 template <int OUT>
 int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float* x, float* y, const int* done,
                 const float* bv = nullptr, const float* dinv = nullptr, float* dv = nullptr, float c1 = 0.f,
-                float c2 = 0.f, double* partial = nullptr, int cflags = CH_READ_D | CH_WRITE_D) {
+                float c2 = 0.f, double* partial = nullptr, int cflags = CH_READ_D | CH_WRITE_D,
+                SolverBatch* tb = nullptr) {
     if (T.n_bricks == 0) return DTC_OK;
     MvArgs a{T.bricks, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2, cflags};
     const dim3 grid(T.n_bricks, chunk_split(T.n_bricks, R / MvCfg<float>::RC));
+    constexpr int kind = OUT == OUT_CHEB ? DTC_KSTAT_CHEB : OUT == OUT_BRES ? DTC_KSTAT_BRES : -1;
+    int slot = -1;
+    if (tb && kind >= 0) DTC_TRY(timed_begin(*tb, kind, &slot));
     k_matvec<float, IN_V, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
     DTC_HIP(hipGetLastError());
+    if (tb) {
+        // x, y, b; Chebyshev: the direction read and written as the flags say
+        const int vp = 3 + (OUT == OUT_CHEB && (cflags & CH_READ_D) ? 1 : 0) +
+                       (OUT == OUT_CHEB && (cflags & CH_WRITE_D) ? 1 : 0);
+        DTC_TRY(timed_end(*tb, slot, vp, false, OUT == OUT_CHEB, done));
+    }
     return DTC_OK;
 }
 
@@ -2407,7 +2471,7 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
             } else {
                 const int fl = (k == 0 ? 0 : (zero && k == 1) ? CH_D_FROM_X : CH_READ_D) | (k < m - 1 ? CH_WRITE_D : 0);
                 DTC_TRY(launch_mv32<OUT_CHEB>(v.T, s, wk, v.buf[cur[l]], v.buf[1 - cur[l]], done, v.b, v.dinv, v.d,
-                                              c1, c2, dk, fl));
+                                              c1, c2, dk, fl, l == 0 ? &b : nullptr));
                 cur[l] = 1 - cur[l];
             }
         }
@@ -2422,7 +2486,8 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
         cur[l + 1] = 0;
         if (c.T.n_bricks == 0 || f.T.n_bricks == 0) continue;
         float* res = f.buf[1 - cur[l]];
-        DTC_TRY(launch_mv32<OUT_BRES>(f.T, s, wk, f.buf[cur[l]], res, done, f.b));
+        DTC_TRY(launch_mv32<OUT_BRES>(f.T, s, wk, f.buf[cur[l]], res, done, f.b, nullptr, nullptr, 0.f, 0.f, nullptr,
+                                      0, l == 0 ? &b : nullptr));
         k_restrict<<<dim3(c.T.n_bricks, node_split(c.T.n_bricks)), 256, 0, s>>>(c.T.bricks, c.T.doms, c.l2n, f.T.doms, f.l2n, res, done, c.b);
         DTC_HIP(hipGetLastError());
     }
@@ -2492,16 +2557,13 @@ int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* di
 int launch_mv_q64(SolverBatch& b, const float* p, float* q, const int* done) {
     MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(),
              b.einfo.as<int>(), b.eE64.p, done, nullptr};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (b.timing) {
-        DTC_TRY(event_pair(b, &e0, &e1));
-        DTC_HIP(hipEventRecord(e0, b.stream));
-    }
+    int slot = -1;
+    DTC_TRY(timed_begin(b, DTC_KSTAT_MV, &slot));
     const dim3 grid(b.n_bricks, chunk_split(b.n_bricks, R / MvCfg<double>::RC));
     k_matvec<double, IN_V, OUT_MASKED, true><<<grid, MV_NT, 0, b.stream>>>(
         a, cast_wk<double>(b.wk64), reinterpret_cast<const double*>(p), reinterpret_cast<double*>(q));
     DTC_HIP(hipGetLastError());
-    if (b.timing) DTC_HIP(hipEventRecord(e1, b.stream));
+    DTC_TRY(timed_end(b, slot, 2, false, false, done));
     return DTC_OK;
 }
 
@@ -2521,12 +2583,12 @@ int block_setup(SolverBatch& b) {
             sl.push_back(make_int4(d, (int)n0, (int)std::min<int64_t>(n0 + step, D.node_end), 0));
     }
     ds[b.n_dom] = (int)sl.size();
-    DTC_TRY(b.slices.alloc(std::max<size_t>(sl.size(), 1) * sizeof(int4)));
-    DTC_TRY(b.dslice.alloc(ds.size() * 4));
-    DTC_TRY(b.gpart.alloc(std::max<size_t>(sl.size(), 1) * 2 * RR * 8));
-    DTC_TRY(b.gsum.alloc((size_t)std::max(b.n_dom, 1) * 2 * RR * 8));
-    DTC_TRY(b.lfac.alloc((size_t)std::max(b.n_dom, 1) * RR * 8));
-    DTC_TRY(b.bcoef.alloc((size_t)std::max(b.n_dom, 1) * RR * 4));
+    DTC_TRY(b.slices.ensure(std::max<size_t>(sl.size(), 1) * sizeof(int4)));
+    DTC_TRY(b.dslice.ensure(ds.size() * 4));
+    DTC_TRY(b.gpart.ensure(std::max<size_t>(sl.size(), 1) * 2 * RR * 8));
+    DTC_TRY(b.gsum.ensure((size_t)std::max(b.n_dom, 1) * 2 * RR * 8));
+    DTC_TRY(b.lfac.ensure((size_t)std::max(b.n_dom, 1) * RR * 8));
+    DTC_TRY(b.bcoef.ensure((size_t)std::max(b.n_dom, 1) * RR * 4));
     if (!sl.empty())
         DTC_HIP(hipMemcpyAsync(b.slices.p, sl.data(), sl.size() * sizeof(int4), hipMemcpyHostToDevice, b.stream));
     DTC_HIP(hipMemcpyAsync(b.dslice.p, ds.data(), ds.size() * 4, hipMemcpyHostToDevice, b.stream));
@@ -2604,7 +2666,7 @@ int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p,
 // then per load case activity and targets (k_red_outer)
 int residual64(SolverBatch& b, const WK<double>& wk, const double* x64, int first, const dtc_solve_opts& o) {
     DTC_TRY((launch_mv<double, IN_U, OUT_RESID>(b, wk, x64, b.r64.as<double>(), b.part_c.as<double>(), nullptr)));
-    const double itol2 = b.precision == DTC_FP32_REFINE ? o.inner_tol * o.inner_tol : 0.0;
+    const double* itol2 = b.precision == DTC_FP32_REFINE ? b.itol2.as<double>() : nullptr;
     k_red_outer<<<b.n_dom, RED_NT, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_c.as<double>(), first,
                                                   o.rel_tol * o.rel_tol, itol2, b.rr0.as<double>(), b.rr.as<double>(),
                                                   b.active64.as<int>(), b.active.as<int>(), b.thr2.as<double>(),
@@ -2697,14 +2759,27 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
     const char* be = std::getenv("DTC_BLOCK_PCG");
     const bool block = b.precision == DTC_FP32_REFINE && b.precond == DTC_PC_MG && !b.micro20 &&
                        !(be && be[0] == '0');
-    // inner_tol outside (0, 1): automatic.  The block restarts at every FP64
-    // residual replacement, so it takes its FP32 phases as deep as FP32 allows:
-    // 1e-6 (two phases to 1e-8 instead of three) where the voxel moduli differ by
-    // at most 100x; high-contrast (greyscale) domains stall in FP32 before that
-    // and use 1e-4, as does one PCG per load case (it keeps its direction over a
-    // replacement)
-    dtc_solve_opts o = opts;
-    if (!(o.inner_tol > 0.0 && o.inner_tol < 1.0)) o.inner_tol = block && b.contrast <= 100.0 ? 1e-6 : 1e-4;
+    // inner_tol outside (0, 1): automatic, per domain.  The block restarts at every
+    // FP64 residual replacement, so it takes its FP32 phases as deep as FP32
+    // allows: 1e-6 (two phases to 1e-8 instead of three) where the moduli of the
+    // domain's solid voxels differ by at most 100x; high-contrast (greyscale)
+    // domains stall in FP32 before that and use 1e-4, as does one PCG per load
+    // case (it keeps its direction over a replacement)
+    const dtc_solve_opts& o = opts;
+    if (D > 0) {
+        std::vector<double> it2(D);
+        for (int d = 0; d < D; ++d) {
+            double t = o.inner_tol;
+            if (!(t > 0.0 && t < 1.0)) {
+                const double c = d < (int)b.h_contrast.size() ? b.h_contrast[d] : 1.0;
+                t = block && c <= 100.0 ? 1e-6 : 1e-4;
+            }
+            it2[d] = t * t;
+        }
+        DTC_TRY(b.itol2.ensure((size_t)D * 8));
+        DTC_HIP(hipMemcpyAsync(b.itol2.p, it2.data(), (size_t)D * 8, hipMemcpyHostToDevice, b.stream));
+        DTC_HIP(hipStreamSynchronize(b.stream));
+    }
     iters.assign(D, 0);
     outer.assign(D, 0);
     relres.assign(D, 0.0);
@@ -2726,9 +2801,25 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
         k_fill_f32<<<nb(nv, 256), 256, 0, s>>>(b.q32.as<float>(), nv, 0.f);
     }
     DTC_HIP(hipGetLastError());
-    b.mv_ms_sum = 0.0;
-    b.mv_count = 0;
+    for (KernelStat& k : b.kstat) k = KernelStat{};
     b.reactions_valid = false;
+    if (b.timing) {
+        // per domain group: vector bytes (FP32), moduli (FP32), Jacobi scaling (FP32), vector bytes (FP64)
+        std::vector<double> db((size_t)D * 4, 0.0);
+        for (int d = 0; d < D; ++d) {
+            const double nn = (double)(b.h_dom[d].node_end - b.h_dom[d].node_begin);
+            const double ne = b.h_status[d] == DTC_OK ? (double)b.h_elems[d] : 0.0;
+            db[4 * d] = 3.0 * nn * R * 4.0;
+            db[4 * d + 1] = 4.0 * ne;
+            db[4 * d + 2] = 3.0 * nn * 4.0;
+            db[4 * d + 3] = 3.0 * nn * R * 8.0;
+        }
+        DTC_TRY(b.dom_bytes.ensure(db.size() * 8));
+        DTC_TRY(b.launch_bytes.ensure((size_t)MAX_SLOTS * 8));
+        DTC_HIP(hipMemcpyAsync(b.dom_bytes.p, db.data(), db.size() * 8, hipMemcpyHostToDevice, s));
+        DTC_HIP(hipStreamSynchronize(s));
+        b.n_slots = 0;
+    }
     std::vector<int64_t> nouter(D, 0);
     DTC_TRY(residual64(b, wk64, nullptr, 1, o));          // r64 = f (lifted right-hand side)
 

@@ -4,6 +4,12 @@
 
 namespace dtc {
 
+// Accumulated timing of one timed kernel variant.
+struct KernelStat {
+    double ms_sum = 0.0, bytes_sum = 0.0;
+    int64_t count = 0;
+};
+
 // One coarse level of the multigrid preconditioner: voxels of 2^l fine voxels,
 // same brick layout and matvec tables as the fine level, FP32 vectors.
 struct Level {
@@ -96,15 +102,22 @@ struct SolverBatch {
     WalshK wk64{};
     double kdiag = 0.0;            // diagonal entry of the E = 1 element stiffness
     double nu = 0.3, a = 1.0;
-    double contrast = 1.0;         // per-voxel modulus contrast (max / min) of the batch
+    std::vector<double> h_contrast;   // per domain group: modulus contrast (max / min) over its solid voxels
+    DevMem itol2;                  // double [dom]: squared inner (FP32 phase) reduction of each domain group
     int precision = DTC_FP64;
     bool reactions_valid = false;
     bool capturing = false;
     hipStream_t stream = nullptr;
-    // matvec timing
-    std::vector<hipEvent_t> ev, ev_free, ev_used;
-    double mv_ms_sum = 0.0;
-    int64_t mv_count = 0;
+    // kernel timing (off unless enabled): HIP events around the fine-level
+    // launches of each timed kernel, and the algorithmic bytes of each launch
+    // summed on the device over the domains still iterating (k_launch_bytes)
+    std::vector<hipEvent_t> ev, ev_free;
+    struct Timed { hipEvent_t e0, e1; int kind; int slot; };
+    std::vector<Timed> ev_used;
+    DevMem dom_bytes;              // double [dom][4]: vector bytes V, modulus bytes, Jacobi bytes, FP64 V
+    DevMem launch_bytes;           // double [slot]: algorithmic bytes of a timed launch
+    int n_slots = 0;
+    KernelStat kstat[DTC_KSTAT_COUNT];
     bool timing = false;
 };
 

@@ -25,7 +25,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (DTC_ERR_EMPTY_MESH, DTC_ERR_FEW_CONSTRAINTS, DTC_ERR_NO_CONVERGENCE, DTC_FP32_REFINE, DTC_FP64,
-                   DTC_PC_JACOBI, DTC_PC_MG, DTC_OK, DTCError, DomainDesc, DomainResult, SolveOpts, check, load)
+                   DTC_KSTAT_BRES, DTC_KSTAT_CHEB, DTC_KSTAT_MV, DTC_KSTAT_RESID, DTC_PC_JACOBI, DTC_PC_MG, DTC_OK,
+                   KSTAT_NAMES, DTCError, DomainDesc, DomainResult, SolveOpts, check, load)
 
 KIND = {np.dtype(np.int8): 1, np.dtype(np.int16): 2, np.dtype(np.int32): 4}
 
@@ -199,7 +200,8 @@ def _lcs(macro_order):
 def _result(r: DomainResult, macro_order=1) -> dict:
     a = lambda x, *s: np.array(x[:int(np.prod(s))], dtype=np.float64).reshape(*s)
     R = _lcs(macro_order)
-    out = dict(status=r.status, n_elems=r.n_elems, n_nodes=r.n_nodes, n_bnodes=r.n_bnodes, iterations=r.iterations,
+    out = dict(status=r.status, precision_used=r.precision_used, n_elems=r.n_elems, n_nodes=r.n_nodes,
+               n_bnodes=r.n_bnodes, iterations=r.iterations,
                outer_iterations=r.outer_iterations, relres=r.relres_max,
                ff=a(r.ff, R, R).T.copy(), K_macro=a(r.K_macro, R, R).T.copy(), C=a(r.C, 6, 6).T.copy(),
                EE=a(r.EE, 6, 6).T.copy(), density=r.density)
@@ -238,6 +240,18 @@ class Batch:
         check("dtc_batch_stats", load().dtc_batch_stats(self._h, C.byref(avg), C.byref(n), C.byref(by), C.byref(fl),
                                                         C.byref(run)))
         return dict(mv_avg_ms=avg.value, mv_launches=n.value, mv_bytes=by.value, mv_flops=fl.value, run_ms=run.value)
+
+    def timing(self, enable=True):
+        """Kernel timing (HIP events on the solve stream) of the following runs on/off."""
+        check("dtc_batch_timing", load().dtc_batch_timing(self._h, int(bool(enable))))
+
+    def kernel_stats(self, kernel):
+        """Average launch duration (ms), launches and average algorithmic bytes of one
+        timed kernel (DTC_KSTAT_MV / _CHEB / _BRES / _RESID) over the last run."""
+        avg, n, by = C.c_double(), C.c_int64(), C.c_double()
+        check("dtc_batch_kernel_stats", load().dtc_batch_kernel_stats(self._h, int(kernel), C.byref(avg), C.byref(n),
+                                                                      C.byref(by)))
+        return dict(avg_ms=avg.value, launches=n.value, bytes=by.value)
 
     def close(self):
         if self._h:

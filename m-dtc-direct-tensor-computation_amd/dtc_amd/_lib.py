@@ -24,6 +24,11 @@ DTC_FP64 = 0
 DTC_FP32_REFINE = 1
 DTC_PC_JACOBI = 0
 DTC_PC_MG = 1
+DTC_KSTAT_MV = 0
+DTC_KSTAT_CHEB = 1
+DTC_KSTAT_BRES = 2
+DTC_KSTAT_RESID = 3
+KSTAT_NAMES = {DTC_KSTAT_MV: "mv", DTC_KSTAT_CHEB: "cheb", DTC_KSTAT_BRES: "bres", DTC_KSTAT_RESID: "resid"}
 
 I64 = C.c_int64
 P = C.c_void_p
@@ -49,7 +54,7 @@ class SolveOpts(C.Structure):
 
 class DomainResult(C.Structure):
     _fields_ = [
-        ("status", C.c_int), ("_pad", C.c_int), ("n_elems", I64), ("n_nodes", I64), ("n_bnodes", I64),
+        ("status", C.c_int), ("precision_used", C.c_int), ("n_elems", I64), ("n_nodes", I64), ("n_bnodes", I64),
         ("iterations", I64), ("outer_iterations", I64), ("relres_max", C.c_double),
         ("ff", C.c_double * 3600), ("K_macro", C.c_double * 3600), ("C", C.c_double * 36), ("EE", C.c_double * 36),
         ("density", C.c_double), ("rot_alpha", C.c_double * 2), ("rot_n", C.c_double * 6),
@@ -74,6 +79,8 @@ SIGNATURES = {
     "dtc_batch_run": (C.c_int, [P, P]),
     "dtc_batch_grad": (C.c_int, [P, I64, P, P]),
     "dtc_batch_stats": (C.c_int, [P, P, P, P, P, P]),
+    "dtc_batch_timing": (C.c_int, [P, C.c_int]),
+    "dtc_batch_kernel_stats": (C.c_int, [P, C.c_int, P, P, P]),
     "dtc_batch_destroy": (C.c_int, [P]),
     "dtc_rotation_search": (C.c_int, [P, C.c_int, C.c_double, C.c_double, P, P, P, P, P, P]),
 }

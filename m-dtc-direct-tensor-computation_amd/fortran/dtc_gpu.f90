@@ -54,7 +54,7 @@ module dtc_gpu
   end type dtc_solve_opts
 
   type, bind(c) :: dtc_domain_result
-     integer(c_int)       :: status, pad
+     integer(c_int)       :: status, precision_used
      integer(c_int64_t)   :: n_elems, n_nodes, n_bnodes, iterations, outer_iterations
      real(c_double)       :: relres_max
      real(c_double)       :: ff(3600), K_macro(3600)   ! column-major R x R, R = 24 or 60

@@ -1,0 +1,107 @@
+"""Batch handle controls on the GPU: reuse of one handle under a different
+sub-batch split, the per-domain automatic inner tolerance, the precision a
+result was produced in, and the kernel timing statistics."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _dtc():
+    import dtc_amd
+    return dtc_amd
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _binary(dtc, n, seed):
+    from dtc_amd import synth
+    return dtc.Domain(synth.trabecular(n, seed=seed, sigma=1.5, bvtv=0.3), delta=(0.01,) * 3)
+
+
+def _grey(dtc, n, seed):
+    from dtc_amd import synth
+    g, m = synth.greyscale(n, seed=seed, bvtv=0.3)
+    return dtc.Domain(g, llimit=1, delta=(0.01,) * 3, modulus=m, E=10000.0)
+
+
+def test_one_handle_run_concurrent_then_sequential(monkeypatch):
+    """The same Batch handle run first as two concurrent halves, then as one
+    sub-batch, then cut into small sub-batches: the block PCG's per-domain slices
+    follow each split, so all three runs give the same results."""
+    dtc = _dtc()
+    doms = [_binary(dtc, 16 + 2 * i, 40 + i) for i in range(4)]
+    kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+    with dtc.Batch(doms, **kw) as b:
+        monkeypatch.setenv("DTC_CONCURRENT", "2")
+        r2 = b.run()
+        monkeypatch.setenv("DTC_CONCURRENT", "1")
+        r1 = b.run()
+        monkeypatch.setenv("DTC_SUBBATCH_NODES", "6000")
+        rs = b.run()
+    for a, c, d in zip(r2, r1, rs):
+        assert a["status"] == c["status"] == d["status"] == 0
+        assert a["iterations"] == c["iterations"] == d["iterations"]
+        assert _rel(a["C"], c["C"]) < 1e-12 and _rel(d["C"], c["C"]) < 1e-12
+        assert _rel(a["K_macro"], c["K_macro"]) < 1e-12
+
+
+def test_inner_tolerance_is_chosen_per_domain():
+    """Automatic inner tolerance: 1e-6 for a binary domain, 1e-4 for a greyscale
+    one (contrast > 100 over its solid voxels).  In a batch that mixes both, each
+    domain iterates exactly as when solved alone, and as with its tolerance given
+    explicitly."""
+    dtc = _dtc()
+    bi, gr = _binary(dtc, 24, 51), _grey(dtc, 20, 52)
+    kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+    mix = dtc.solve_batch([bi, gr], **kw)
+    b_alone = dtc.solve_domain(bi, **kw)
+    g_alone = dtc.solve_domain(gr, **kw)
+    b_6 = dtc.solve_domain(bi, inner_tol=1e-6, **kw)
+    g_4 = dtc.solve_domain(gr, inner_tol=1e-4, **kw)
+    for r in mix + [b_alone, g_alone, b_6, g_4]:
+        assert r["status"] == 0 and r["relres"] <= 1e-8
+    key = lambda r: (r["iterations"], r["outer_iterations"])
+    assert key(mix[0]) == key(b_alone) == key(b_6)
+    assert key(mix[1]) == key(g_alone) == key(g_4)
+    assert _rel(mix[0]["C"], b_alone["C"]) < 1e-12 and _rel(mix[1]["C"], g_alone["C"]) < 1e-12
+
+
+def test_precision_used_flag():
+    dtc = _dtc()
+    d = _binary(dtc, 12, 61)
+    r1 = dtc.solve_domain(d, precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+    r0 = dtc.solve_domain(d, precision=0, rel_tol=1e-8, rotation=False)
+    assert r1["status"] == 0 == r0["status"]
+    assert r1["precision_used"] == dtc.DTC_FP32_REFINE and r0["precision_used"] == dtc.DTC_FP64
+
+
+def test_kernel_timing_statistics():
+    """With timing on, the CG matvec, the fused Chebyshev step, the restriction
+    residual and the FP64 residual are timed; the average algorithmic bytes of a
+    launch lie between one domain's and the whole batch's, for the formula of each
+    kernel.  With timing off nothing is timed."""
+    dtc = _dtc()
+    doms = [_binary(dtc, 20, 70 + i) for i in range(3)]
+    with dtc.Batch(doms, precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG) as b:
+        res = b.run()
+        assert all(b.kernel_stats(k)["launches"] == 0 for k in dtc.KSTAT_NAMES)
+        b.timing(True)
+        res = b.run()
+        st = {k: b.kernel_stats(k) for k in dtc.KSTAT_NAMES}
+    assert all(r["status"] == 0 for r in res)
+    V = [3 * r["n_nodes"] * 24 * 4 for r in res]
+    E = [4 * r["n_elems"] for r in res]
+    J = [3 * r["n_nodes"] * 4 for r in res]
+    one = lambda vp, jac=False, s=1: min(vp * s * v + s * e + (j if jac else 0) for v, e, j in zip(V, E, J))
+    full = lambda vp, jac=False, s=1: sum(vp * s * v + s * e + (j if jac else 0) for v, e, j in zip(V, E, J))
+    for k, (lo, hi) in {dtc.DTC_KSTAT_MV: (one(2), full(2)), dtc.DTC_KSTAT_BRES: (one(3), full(3)),
+                        dtc.DTC_KSTAT_CHEB: (one(3, True), full(5, True)),
+                        dtc.DTC_KSTAT_RESID: (one(1, s=2), full(2, s=2))}.items():
+        s = st[k]
+        assert s["launches"] > 0 and s["avg_ms"] > 0.0, (k, s)
+        assert lo * (1 - 1e-12) <= s["bytes"] <= hi * (1 + 1e-12), (k, s, lo, hi)
+    # the block PCG runs one Q = K P per iteration, the V-cycle three fused steps
+    assert st[dtc.DTC_KSTAT_CHEB]["launches"] >= 2 * st[dtc.DTC_KSTAT_MV]["launches"]
