@@ -34,7 +34,11 @@ for p in (ROOT, PKG):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-FP32_PEAK_TFLOPS = 157.3       # MI355X dense FP32 matrix
+# the timed kernels (fine-level launches): kernel_stats name -> kernel
+KERNEL_NAMES = {"mv": "k_matvec<float, IN_V, OUT_MASKED> (Q = K P of the block PCG)",
+                "cheb": "k_matvec<float, IN_V, OUT_CHEB> (Chebyshev-Jacobi smoothing step)",
+                "bres": "k_matvec<float, IN_V, OUT_BRES> (b - K x before restriction)",
+                "resid": "k_matvec<double, IN_U, OUT_RESID> (FP64 true residual)"}
 
 
 def _sync():
@@ -53,12 +57,15 @@ def make_domains(count, n, seed0):
                            delta=(0.01, 0.01, 0.01), xa_n=(2, 2, 2), E=5600.0, nu=0.3) for i in range(count)]
 
 
-def cpu_baseline(dom, gpu_fp64_iters_per_lc, sample_iters, log):
-    """The oracle (assembled 3x3-block CSR, Jacobi-PCG per load case, FP64, one
-    core) on a bounded sample: one load case for `sample_iters` PCG iterations
-    plus one CR_1 search, extrapolated to a full domain with the iteration
-    counts of the FP64 solve of the same domain."""
+def _cpu_sample(job):
+    """One core: the oracle (optimised build) on its own domain, bounded sample."""
+    seed, n, iters_per_lc, sample_iters = job
+    os.environ["DTC_ORACLE_FAST"] = "1"
     from oracle import oracle as O
+    from dtc_amd import synth
+    import dtc_amd
+    dom = dtc_amd.Domain(synth.trabecular(n, seed=seed, sigma=2.0, bvtv=0.20), llimit=1, delta=(0.01, 0.01, 0.01),
+                         xa_n=(2, 2, 2), E=5600.0, nu=0.3)
     t0 = time.perf_counter()
     nodes, elems = O.mesh(dom.phi, dom.llimit, 8, dom.delta, dom.xa_n)
     t_mesh = time.perf_counter() - t0
@@ -72,13 +79,44 @@ def cpu_baseline(dom, gpu_fp64_iters_per_lc, sample_iters, log):
     O.rotation_search(np.diag([3000.0, 3100, 3300, 900, 950, 1000]) + 50.0, 1)
     t_rot = time.perf_counter() - t0
     per_iter = t_solve / max(it, 1)
-    t_domain = t_mesh + per_iter * gpu_fp64_iters_per_lc * 24 + 2 * t_rot
-    log(f"cpu sample: mesh {t_mesh:.2f}s, {it} PCG iterations in {t_solve:.2f}s, CR_1 search {t_rot:.2f}s "
-        f"-> {t_domain:.1f}s per domain")
-    return {"value": 1.0 / t_domain, "unit": "domains/s", "cores": 1, "kind": "port",
-            "sample": f"oracle on one 64^3 domain: mesh, load case 1 for {it} Jacobi-PCG iterations "
-                      f"({per_iter * 1e3:.2f} ms each), one CR_1 search; extrapolated to 24 load cases x "
-                      f"{gpu_fp64_iters_per_lc:.0f} iterations (the FP64 iteration count) + CR_1 and CR_2"}
+    return t_mesh, per_iter, t_rot, it
+
+
+def cpu_baseline(n, gpu_fp64_iters_per_lc, sample_iters, workers, log):
+    """The oracle (assembled 3x3-block CSR, Jacobi-PCG per load case, FP64; the
+    optimised build of oracle/Makefile) with one domain per host core, all cores
+    at once (the reference runs one domain per MPI worker), on a bounded sample:
+    each core meshes its domain, runs load case 1 for `sample_iters` PCG
+    iterations and one CR_1 search; a domain's time is extrapolated with the
+    FP64 iteration count per load case x 24 load cases + CR_1 and CR_2, and the
+    host's rate is the sum of the per-core rates."""
+    import multiprocessing as mp
+    import platform
+    t0 = time.perf_counter()
+    jobs = [(2 + i % 16, n, gpu_fp64_iters_per_lc, sample_iters) for i in range(workers)]
+    with mp.get_context("spawn").Pool(workers) as pool:
+        out = pool.map(_cpu_sample, jobs)
+    wall = time.perf_counter() - t0
+    t_dom = [tm + pi * gpu_fp64_iters_per_lc * 24 + 2 * tr for tm, pi, tr, _ in out]
+    rate = sum(1.0 / t for t in t_dom)
+    per_iter = float(np.mean([o[1] for o in out]))
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        cpu = platform.processor() or cpu
+    log(f"cpu sample: {workers} cores ({cpu}), {out[0][3]} PCG iterations each, {per_iter * 1e3:.2f} ms per "
+        f"iteration, {np.mean(t_dom):.1f} s per domain extrapolated, {wall:.1f} s wall")
+    return {"value": rate, "unit": "domains/s", "cores": workers, "kind": "port", "cpu_model": cpu,
+            "host_cpus": os.cpu_count(),
+            "sample": f"oracle (gcc -O3 -march=x86-64-v3) on {workers} cores at once, one 64^3 domain each (seeds "
+                      f"2..): mesh, load case 1 for {out[0][3]} Jacobi-PCG iterations ({per_iter * 1e3:.2f} ms "
+                      f"each on average), one CR_1 search; each domain extrapolated to 24 load cases x "
+                      f"{gpu_fp64_iters_per_lc:.0f} iterations (the FP64 GPU iteration count) + CR_1 and CR_2; "
+                      f"{wall:.1f} s of sampling"}
 
 
 def uneven(args, log):
@@ -123,6 +161,8 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--fp64-domains", type=int, default=2)
     ap.add_argument("--cpu-iters", type=int, default=400)
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host cores of the CPU baseline (one domain each; 16 = one GPU's share of the host)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=200000, help="PCG iteration cap (profiling runs)")
     ap.add_argument("--no-check", action="store_true", help="do not require convergence (profiling runs)")
@@ -149,7 +189,7 @@ def main():
     log(f"batch of {len(doms)} domains of {args.n}^3, BV/TV " +
         ", ".join(f"{d.phi.mean():.3f}" for d in doms[:4]) + " ...")
 
-    # ---- batched FP32 + FP64 refinement
+    # ---- batched FP32 + FP64 refinement (kernel timing off in the timed steps)
     pc = dtc_amd.DTC_PC_MG if args.precond == "mg" else dtc_amd.DTC_PC_JACOBI
     b = dtc_amd.Batch(doms, precision=dtc_amd.DTC_FP32_REFINE, rel_tol=1e-8, inner_tol=args.inner_tol, rotation=True,
                       max_iter=args.max_iter, precond=pc, mg_degree=args.mg_degree)
@@ -163,14 +203,37 @@ def main():
         log(f"step {k}: {b.stats()['run_ms']:.1f} ms")
     _sync()
     dt = time.perf_counter() - t0
-    st = b.stats()
     bad = [i for i, r in enumerate(res) if r["status"] != 0 or r["relres"] > 1e-8]
     if bad and not args.no_check:
         raise SystemExit(f"domains {bad} did not converge to 1e-8: {[res[i]['status'] for i in bad]}")
     value = len(doms) * args.steps / dt
     mixed_iters = np.mean([r["iterations"] for r in res])
+    fallbacks = sum(1 for r in res if r["precision_used"] != dtc_amd.DTC_FP32_REFINE)
     log("PCG iterations per domain: " + " ".join(str(r["iterations"]) for r in res) +
-        "; FP64 residual replacements: " + " ".join(str(r["outer_iterations"]) for r in res))
+        "; FP64 residual replacements: " + " ".join(str(r["outer_iterations"]) for r in res) +
+        f"; domains repeated in FP64: {fallbacks}")
+
+    # ---- kernel timing: one more run (outside the timed steps) with HIP events on the
+    # solve stream around the fine-level launches of each timed kernel
+    b.timing(True)
+    b.run()
+    b.timing(False)
+    kern = {}
+    for k, name in dtc_amd.KSTAT_NAMES.items():
+        ks = b.kernel_stats(k)
+        if ks["launches"] == 0 or ks["avg_ms"] <= 0:
+            continue
+        gbs = ks["bytes"] / (ks["avg_ms"] * 1e-3) / 1e9
+        kern[name] = {"kernel": KERNEL_NAMES[name], "avg_launch_ms": round(ks["avg_ms"], 5),
+                      "launches": ks["launches"], "total_ms": round(ks["avg_ms"] * ks["launches"], 2),
+                      "algorithmic_bytes": round(ks["bytes"]), "achieved_GBs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    run_ms = b.stats()["run_ms"]
+    for v in kern.values():
+        v["share_of_run"] = round(v["total_ms"] / run_ms, 3)
+    log("timed kernels: " + "; ".join(f"{k}: {v['avg_launch_ms'] * 1e3:.1f} us x {v['launches']}, "
+                                      f"{v['achieved_GBs']:.0f} GB/s, {100 * v['share_of_run']:.1f}% of the run"
+                                      for k, v in kern.items()))
 
     # ---- sensitivity dC/dE of every domain (reuses the stored load-case solutions)
     G = np.eye(6) + 0.1 * np.ones((6, 6))
@@ -182,42 +245,57 @@ def main():
     _sync()
     t_grad = time.perf_counter() - g0
     log(f"gradient of all {len(doms)} domains: {t_grad * 1e3:.1f} ms ({t_grad / (dt / args.steps):.3f} x one step)")
+    b.close()
 
-    # ---- plain FP64 GPU form, one domain at a time
-    t64, C64, it64 = 0.0, [], []
-    for i in range(min(args.fp64_domains, len(doms))):
-        with dtc_amd.Batch([doms[i]], precision=dtc_amd.DTC_FP64, rel_tol=1e-8, rotation=True,
-                           max_iter=args.max_iter) as b64:
-            b64.run()
-            _sync()
-            s0 = time.perf_counter()
-            r64 = b64.run()[0]
-            _sync()
-            t64 += time.perf_counter() - s0
-        C64.append(r64["C"])
-        it64.append(r64["iterations"])
-        log(f"fp64 domain {i}: {r64['iterations']} iterations, relres {r64['relres']:.2e}")
-    fp64_rate = len(C64) / t64 if C64 else float("nan")
+    # ---- plain FP64 GPU form (Jacobi-PCG in FP64), one domain at a time; beside it
+    # the mixed form with the same Jacobi preconditioner and one PCG per load case, so
+    # that the gain of the precision alone is separated from the multigrid / block gain
+    def one_by_one(prec, precond, count):
+        t, Cs, its = 0.0, [], []
+        for i in range(min(count, len(doms))):
+            with dtc_amd.Batch([doms[i]], precision=prec, rel_tol=1e-8, rotation=True, max_iter=args.max_iter,
+                               precond=precond) as b1:
+                b1.run()
+                _sync()
+                s0 = time.perf_counter()
+                r1 = b1.run()[0]
+                _sync()
+                t += time.perf_counter() - s0
+            Cs.append(r1["C"])
+            its.append(r1["iterations"])
+            log(f"{'fp64' if prec == dtc_amd.DTC_FP64 else 'fp32+fp64 jacobi'} domain {i}: {r1['iterations']} "
+                f"iterations, relres {r1['relres']:.2e}, {r1['status']}")
+        return (len(Cs) / t if Cs else float("nan")), Cs, its
+    fp64_rate, C64, it64 = one_by_one(dtc_amd.DTC_FP64, dtc_amd.DTC_PC_JACOBI, args.fp64_domains)
+    jac_rate, _, itj = one_by_one(dtc_amd.DTC_FP32_REFINE, dtc_amd.DTC_PC_JACOBI, args.fp64_domains)
     terr = max((np.abs(res[i]["C"] - C64[i]).max() / np.abs(C64[i]).max() for i in range(len(C64))), default=float("nan"))
 
-    # ---- roofline of the dominant kernel (EbE matvec Q = A P of the FP32 block PCG)
-    mv_s = st["mv_avg_ms"] * 1e-3
-    achieved = st["mv_bytes"] / mv_s / 1e9 if mv_s > 0 else 0.0
-    traffic = None
-    tj = os.path.join(ROOT, "profiles", "k_matvec_traffic.json")
-    if os.path.exists(tj):
-        t = json.load(open(tj))
-        if t.get("batch") == len(doms) and t.get("domain_edge") == args.n:
-            traffic = t["traffic_bytes_per_dispatch"]
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_matvec<float, IN_V, OUT_MASKED>", "avg_launch_ms": round(st["mv_avg_ms"], 5),
-            "launches_timed": st["mv_launches"], "algorithmic_bytes": st["mv_bytes"],
-            "dense_equiv_tflops": round(st["mv_flops"] / mv_s / 1e12, 2) if mv_s > 0 else 0.0}
+    # ---- roofline of the dominant kernel (largest share of the run)
+    dom_k = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
+    roof = None
+    if dom_k:
+        kd = kern[dom_k]
+        traffic = None
+        tj = os.path.join(ROOT, "profiles", f"traffic_{dom_k}.json")
+        if os.path.exists(tj):
+            t = json.load(open(tj))
+            if t.get("batch") == len(doms) and t.get("domain_edge") == args.n:
+                traffic = t["traffic_bytes_per_dispatch"]
+                kd["traffic"] = traffic
+        roof = {"bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": kd["frac"], "traffic": traffic, "kernel": kd["kernel"],
+                "avg_launch_ms": kd["avg_launch_ms"], "launches_timed": kd["launches"],
+                "algorithmic_bytes": kd["algorithmic_bytes"], "share_of_run": kd["share_of_run"]}
+    for k, v in kern.items():
+        tj = os.path.join(ROOT, "profiles", f"traffic_{k}.json")
+        if "traffic" not in v and os.path.exists(tj):
+            t = json.load(open(tj))
+            if t.get("batch") == len(doms) and t.get("domain_edge") == args.n:
+                v["traffic"] = t["traffic_bytes_per_dispatch"]
 
     cpu = None
     if not args.no_cpu and it64:
-        cpu = cpu_baseline(doms[0], float(np.mean(it64)), args.cpu_iters, log)
+        cpu = cpu_baseline(args.n, float(np.mean(it64)), args.cpu_iters, args.cpu_workers, log)
 
     out = {
         "metric": "converged 64³ domains/sec @rel-res 1e-8: batched FP32+FP64-refine vs plain FP64",
@@ -230,10 +308,15 @@ def main():
                    "preconditioner": "multigrid V-cycle (Chebyshev-Jacobi)" if args.precond == "mg" else "Jacobi",
                    "fp64_form": "Jacobi-PCG in FP64, one domain per batch"},
         "fp64_plain_domains_per_s": round(fp64_rate, 3), "speedup_vs_fp64": round(value / fp64_rate, 2),
+        "fp32_refine_jacobi_domains_per_s": round(jac_rate, 3),
+        "speedup_precision_only": round(jac_rate / fp64_rate, 2),
+        "speedup_preconditioner_and_batch": round(value / jac_rate, 2),
         "tensor_rel_err_vs_fp64": float(f"{terr:.3e}"), "pcg_iters_mixed": float(mixed_iters),
         "pcg_iters_fp64": float(np.mean(it64)) if it64 else None,
+        "pcg_iters_fp32_jacobi": float(np.mean(itj)) if itj else None,
+        "fp64_fallback_domains": fallbacks,
         "grad_over_forward": round(t_grad / (dt / args.steps), 4),
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kern, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -1,6 +1,6 @@
 """Per-kernel totals of a rocprofv3 kernel trace, grouped by kernel and grid.
 
-    python -m dtc_amd.trace_stats <dir>/run_results.db [rows]
+    python bench/trace_stats.py <dir>/run_results.db [rows]
 
 Reads the `kernels` view of the trace database written by
 `rocprofv3 --kernel-trace -d <dir> -o run -- <program>` and prints calls,

@@ -14,6 +14,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libdtc_oracle.so")
+# DTC_ORACLE_FAST=1 (the CPU baseline's timing runs): the -O3 build of the same source
+if os.environ.get("DTC_ORACLE_FAST") == "1":
+    _LIB_PATH = os.path.join(_HERE, "_build", "libdtc_oracle_fast.so")
 _lib = None
 
 KIND = {np.dtype(np.int8): 1, np.dtype(np.int16): 2, np.dtype(np.int32): 4}

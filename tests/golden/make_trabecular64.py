@@ -10,7 +10,7 @@ case, FP64) solves every load case to a true relative residual of 1e-10; the loa
 cases are split over worker processes (each solve is independent).
 
 Writes tests/golden/trabecular64_seed2.npz with the packed mask's SHA-256, the mesh
-sizes, ff (R x R, [lc, dof]), K_M, C, EE, density, and the oracle's iteration
+sizes, ff (R x R, ff[dof, lc] as the reference's ff(dof, lc)), K_M, C, EE, density, and the oracle's iteration
 counts and final relative residuals.  The GPU test regenerates the mask, checks
 the hash, and compares its tensors with these.
 """
@@ -78,7 +78,7 @@ def main():
     relres = np.zeros(24)
     with ProcessPoolExecutor(args.workers) as ex:
         for lc0, cnt, f, it, rr in ex.map(_solve, groups):
-            ff[lc0:lc0 + cnt] = f[lc0:lc0 + cnt]
+            ff[:, lc0:lc0 + cnt] = f[:, lc0:lc0 + cnt]      # ff(dof, lc): one column per load case
             iters[lc0:lc0 + cnt] = it[lc0:lc0 + cnt]
             relres[lc0:lc0 + cnt] = rr[lc0:lc0 + cnt]
             print(f"load cases {lc0}..{lc0 + cnt - 1}: {it[lc0:lc0 + cnt].tolist()} iterations, "

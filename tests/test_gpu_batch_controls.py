@@ -29,23 +29,39 @@ def _grey(dtc, n, seed):
 
 def test_one_handle_run_concurrent_then_sequential(monkeypatch):
     """The same Batch handle run first as two concurrent halves, then as one
-    sub-batch, then cut into small sub-batches: the block PCG's per-domain slices
-    follow each split, so all three runs give the same results."""
+    sub-batch, then cut into small sub-batches: each run equals, bit for bit, a
+    fresh handle run with the same split (the block PCG's per-domain slices and
+    every cached buffer follow the split), and the three splits agree to 1e-7,
+    i.e. to what a relative residual of 1e-8 fixes (a domain's multigrid and
+    launch shapes depend on its sub-batch, so the floating-point order, and an
+    iteration count by one, can differ: measured 4.4e-9)."""
     dtc = _dtc()
     doms = [_binary(dtc, 16 + 2 * i, 40 + i) for i in range(4)]
     kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+    splits = [dict(DTC_CONCURRENT="2"), dict(DTC_CONCURRENT="1"), dict(DTC_CONCURRENT="1", DTC_SUBBATCH_NODES="6000")]
+
+    def env(sp):
+        monkeypatch.delenv("DTC_SUBBATCH_NODES", raising=False)
+        for k, v in sp.items():
+            monkeypatch.setenv(k, v)
+
+    reused = []
     with dtc.Batch(doms, **kw) as b:
-        monkeypatch.setenv("DTC_CONCURRENT", "2")
-        r2 = b.run()
-        monkeypatch.setenv("DTC_CONCURRENT", "1")
-        r1 = b.run()
-        monkeypatch.setenv("DTC_SUBBATCH_NODES", "6000")
-        rs = b.run()
-    for a, c, d in zip(r2, r1, rs):
-        assert a["status"] == c["status"] == d["status"] == 0
-        assert a["iterations"] == c["iterations"] == d["iterations"]
-        assert _rel(a["C"], c["C"]) < 1e-12 and _rel(d["C"], c["C"]) < 1e-12
-        assert _rel(a["K_macro"], c["K_macro"]) < 1e-12
+        for sp in splits:
+            env(sp)
+            reused.append(b.run())
+    for sp, rr in zip(splits, reused):
+        env(sp)
+        with dtc.Batch(doms, **kw) as f:
+            rf = f.run()
+        for a, c in zip(rr, rf):
+            assert a["status"] == c["status"] == 0, sp
+            assert a["iterations"] == c["iterations"] and a["outer_iterations"] == c["outer_iterations"], sp
+            assert np.array_equal(a["C"], c["C"]) and np.array_equal(a["K_macro"], c["K_macro"]), sp
+    for a, c, d in zip(*reused):
+        assert abs(a["iterations"] - c["iterations"]) <= 2 and abs(d["iterations"] - c["iterations"]) <= 2
+        assert _rel(a["C"], c["C"]) < 1e-7 and _rel(d["C"], c["C"]) < 1e-7
+        assert _rel(a["K_macro"], c["K_macro"]) < 1e-7 and _rel(d["K_macro"], c["K_macro"]) < 1e-7
 
 
 def test_inner_tolerance_is_chosen_per_domain():
