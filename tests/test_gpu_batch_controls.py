@@ -82,7 +82,8 @@ def test_inner_tolerance_is_chosen_per_domain():
     key = lambda r: (r["iterations"], r["outer_iterations"])
     assert key(mix[0]) == key(b_alone) == key(b_6)
     assert key(mix[1]) == key(g_alone) == key(g_4)
-    assert _rel(mix[0]["C"], b_alone["C"]) < 1e-12 and _rel(mix[1]["C"], g_alone["C"]) < 1e-12
+    # a domain's multigrid and launch shapes depend on its batch: equal to what rel-res 1e-8 fixes
+    assert _rel(mix[0]["C"], b_alone["C"]) < 1e-7 and _rel(mix[1]["C"], g_alone["C"]) < 1e-7
 
 
 def test_precision_used_flag():
