@@ -272,6 +272,7 @@ struct dtc_batch {
     struct Sub {
         Stream own;                // the sub-batch's stream
         SolverBatch sb;
+        SolverBatch sb08;          // HEX20 micro elements: the HEX08 twin of the p-multigrid
         std::vector<double> ff;
         std::vector<int64_t> iters, outer;
         std::vector<double> relres;
@@ -599,17 +600,43 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
             }
         }
         DTC_TRY(batch_finalize(sb, precision));
-        // the multigrid hierarchy coarsens the voxel lattice: HEX20 micro meshes use Jacobi
-        sb.precond = (precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG && !sb.micro20) ? DTC_PC_MG
-                                                                                               : DTC_PC_JACOBI;
+        // the multigrid preconditioner of the mixed solve: the geometric V-cycle over
+        // the voxel lattice (HEX08 micro), or for HEX20 micro elements a p-multigrid
+        // whose coarse space is the HEX08 twin of the same voxels and domain groups
+        sb.precond = (precision == DTC_FP32_REFINE && b->o.precond == DTC_PC_MG) ? DTC_PC_MG : DTC_PC_JACOBI;
         sb.mg_degree = b->o.mg_degree > 0 ? b->o.mg_degree : 2;
+        sb.p08 = nullptr;
+        if (sb.micro20 && sb.precond == DTC_PC_MG) {
+            SolverBatch& tw = S.sb08;
+            tw.stream = s;
+            tw.nu = b->nu;
+            tw.a = b->a0;
+            tw.micro20 = false;
+            tw.timing = false;
+            DTC_TRY(batch_begin(tw, gd.data(), (int64_t)gd.size()));
+            for (int i : list) {
+                const dtc_domain_desc& d = b->d[i];
+                int rc = mesh_build_device(b->phi[i]->p, d.phi_kind, d.n, d.llimit, 8, b->mesh, s);
+                if (rc == DTC_ERR_EMPTY_MESH) {
+                    for (int g = 0; g < b->ng[i]; ++g) DTC_TRY(batch_add_empty(tw, d, DTC_ERR_EMPTY_MESH, R * g));
+                    continue;
+                }
+                DTC_TRY(rc);
+                const double* mod = b->mod[i]->p ? b->mod[i]->as<double>() : nullptr;
+                for (int g = 0; g < b->ng[i]; ++g) DTC_TRY(batch_add_domain(tw, b->mesh, d, mod, b->escale[i], R * g, s));
+            }
+            DTC_TRY(batch_finalize(tw, DTC_FP32_REFINE));
+        }
         return DTC_OK;
     };
     // one sub-batch, solve: multigrid hierarchy -> solve -> reactions
     auto solve_sub = [&](int k) -> int {
         dtc_batch::Sub& S = *b->subs[k];
         SolverBatch& sb = S.sb;
-        if (sb.precond == DTC_PC_MG) DTC_TRY(batch_build_mg(sb, 8));
+        if (sb.precond == DTC_PC_MG) {
+            if (sb.micro20) DTC_TRY(batch_build_pmg(sb, S.sb08));
+            else DTC_TRY(batch_build_mg(sb, 8));
+        }
         DTC_TRY(batch_solve(sb, b->o, S.iters, S.outer, S.relres));
         DTC_TRY(batch_forces(sb, S.ff));
         return DTC_OK;

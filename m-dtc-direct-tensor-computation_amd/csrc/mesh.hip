@@ -19,7 +19,6 @@
 //                         0.1*delta coordinate tests are exact lattice equalities);
 //   * compaction        = scans over the keep flags, with the last element and
 //                         the last node always kept (:597-651).
-#include <hipcub/hipcub.hpp>
 
 #include "dtc_internal.hpp"
 
@@ -290,12 +289,91 @@ inline unsigned grid_of(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
 
 }  // namespace
 
+namespace {
+
+// Exclusive prefix sums of int32 values: tiles of SCAN_TILE values (256 threads x
+// 4), the tile totals scanned by one workgroup, then every tile's values
+// rescanned with its offset.  Integer sums: exact and independent of the order.
+constexpr int SCAN_NT = 256, SCAN_ITEMS = 4, SCAN_TILE = SCAN_NT * SCAN_ITEMS;
+
+// inclusive scan of one value per thread over the workgroup; returns the
+// workgroup total in *total
+__device__ inline int block_incl_scan(int v, int* s_w, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) s_w[wave] = v;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_NT / 64; ++w) {
+        const int c = s_w[w];
+        if (w < wave) base += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return v + base;
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_tiles(const int* __restrict__ in, int64_t n, int* __restrict__ tile_sum) {
+    __shared__ int s_w[SCAN_NT / 64];
+    const int64_t t0 = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+        if (t0 + k < n) v += in[t0 + k];
+    int tot;
+    (void)block_incl_scan(v, s_w, &tot);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+
+// exclusive scan of the tile totals in place (one workgroup, SCAN_NT per step)
+__global__ __launch_bounds__(SCAN_NT) void k_scan_totals(int* __restrict__ tile_sum, int64_t n_tiles) {
+    __shared__ int s_w[SCAN_NT / 64];
+    int carry = 0;
+    for (int64_t b0 = 0; b0 < n_tiles; b0 += SCAN_NT) {
+        const int64_t i = b0 + threadIdx.x;
+        const int v = i < n_tiles ? tile_sum[i] : 0;
+        int tot;
+        const int inc = block_incl_scan(v, s_w, &tot);
+        if (i < n_tiles) tile_sum[i] = carry + inc - v;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const int* __restrict__ in, int64_t n,
+                                                       const int* __restrict__ tile_off, int* __restrict__ out) {
+    __shared__ int s_w[SCAN_NT / 64];
+    const int64_t t0 = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = t0 + k < n ? in[t0 + k] : 0;
+        sum += v[k];
+    }
+    int tot;
+    int run = block_incl_scan(sum, s_w, &tot) - sum + tile_off[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (t0 + k < n) out[t0 + k] = run;
+        run += v[k];
+    }
+}
+
+}  // namespace
+
 int scan_exclusive_i32(const int* in, int* out, int64_t n, int64_t* total, DevMem& tmp, hipStream_t s) {
     if (n <= 0) { *total = 0; return DTC_OK; }
-    size_t bytes = 0;
-    DTC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s));
-    if (tmp.bytes < bytes) DTC_TRY(tmp.alloc(bytes));
-    DTC_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, (int)n, s));
+    const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    DTC_TRY(tmp.ensure((size_t)tiles * sizeof(int)));
+    k_scan_tiles<<<(unsigned)tiles, SCAN_NT, 0, s>>>(in, n, tmp.as<int>());
+    k_scan_totals<<<1, SCAN_NT, 0, s>>>(tmp.as<int>(), tiles);
+    k_scan_apply<<<(unsigned)tiles, SCAN_NT, 0, s>>>(in, n, tmp.as<int>(), out);
+    DTC_HIP(hipGetLastError());
     int last_in = 0, last_out = 0;
     DTC_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int), hipMemcpyDeviceToHost, s));
     DTC_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(int), hipMemcpyDeviceToHost, s));

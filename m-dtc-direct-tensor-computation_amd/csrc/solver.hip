@@ -30,6 +30,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "bc.hpp"
@@ -605,6 +609,8 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
         A.partial[(int64_t)blockIdx.x * R + tid] = s;
     }
 }
+
+#include "matvec_mf.hpp"
 
 // ---------------------------------------------------------------- vector kernels
 // per brick, thread t keeps load case t % 24
@@ -1805,6 +1811,107 @@ __global__ __launch_bounds__(NT_UPD) void k_cheb(const BrickInfo* __restrict__ b
     }
 }
 
+// ---------------------------------------------------------------- p-multigrid transfers (HEX20 -> HEX08)
+// The HEX20 nodes of a voxel mesh sit on the half-voxel lattice: corners at even
+// coordinates (the HEX08 nodes of the same voxels), mid-edge nodes with one odd
+// coordinate.  P (HEX08 -> HEX20) is the trilinear interpolation of the HEX08
+// element: identity at corners, the mean of the edge's two ends at a mid-edge
+// node.  Restriction is P^T in gather form, in a fixed order per coarse node.
+
+// r08 = P^T res for one domain group: coarse lattice node (i, j, k) takes the fine
+// node at (2i, 2j, 2k) and half of each mid-edge node at (2i +- 1, 2j, 2k),
+// (2i, 2j +- 1, 2k), (2i, 2j, 2k +- 1); 0 on box-face (Dirichlet) nodes.
+__global__ __launch_bounds__(256) void k_pmg_restrict(Dom20Dev G, DomInfo D08, const int* __restrict__ lat2node08,
+                                                     const int* __restrict__ done, int dom,
+                                                     const float* __restrict__ res, float* __restrict__ r08) {
+    if (done && done[dom]) return;
+    const int Lx = D08.nx + 1, Ly = D08.ny + 1, Lz = D08.nz + 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)Lx * Ly * Lz * R) return;
+    const int lc = (int)(i % R);
+    const int64_t p = i / R;
+    const int cx = (int)(p % Lx), cy = (int)((p / Lx) % Ly), cz = (int)(p / ((int64_t)Lx * Ly));
+    const int c = lat2node08[D08.lat_off + p];
+    if (c < 0) return;
+    const int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
+    auto fine = [&](int x, int y, int z) -> int {
+        if (x < 0 || y < 0 || z < 0 || x >= G.Lx || y >= G.Ly || z >= G.Lz) return -1;
+        return G.lat20[x + (int64_t)G.Lx * (y + (int64_t)G.Ly * z)];
+    };
+    const int q0 = fine(fx, fy, fz);
+    const bool bc = q0 >= 0 && G.bcf[q0];
+    const int nb6[6] = {fine(fx - 1, fy, fz), fine(fx + 1, fy, fz), fine(fx, fy - 1, fz),
+                        fine(fx, fy + 1, fz), fine(fx, fy, fz - 1), fine(fx, fy, fz + 1)};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float v = 0.f;
+        if (!bc) {
+            if (q0 >= 0) v = res[((G.node_begin + q0) * 3 + d) * R + lc];
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (nb6[k] >= 0) v += 0.5f * res[((G.node_begin + nb6[k]) * 3 + d) * R + lc];
+        }
+        r08[((int64_t)c * 3 + d) * R + lc] = v;
+    }
+}
+
+// z += P z08 on the free HEX20 nodes of the pseudo-bricks
+__global__ __launch_bounds__(NT_UPD) void k_pmg_prolong(const BrickInfo* __restrict__ bricks,
+                                                       const DomInfo* __restrict__ doms,
+                                                       const Dom20Dev* __restrict__ d20,
+                                                       const DomInfo* __restrict__ doms08,
+                                                       const int* __restrict__ lat2node08,
+                                                       const int* __restrict__ done, const float* __restrict__ z08,
+                                                       float* __restrict__ z) {
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done && done[B.dom]) return;
+    const DomInfo D = doms[B.dom];
+    const DomInfo C = doms08[B.dom];
+    const Dom20Dev G = d20[B.dom];
+    const int Lx = C.nx + 1, Ly = C.ny + 1;
+    for (int i = threadIdx.x; i < (B.node_end - B.node_begin) * R; i += NT_UPD) {
+        const int64_t q = B.node_begin + i / R;
+        const int lc = i % R;
+        const int ql = (int)(q - G.node_begin);
+        if (G.bcf[ql]) continue;
+        int l[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) l[k] = (int)lrint(2.0 * (G.coords[3 * ql + k] / D.delta[k] - D.xa1[k]));
+        int odd = -1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (l[k] & 1) odd = k;
+        int lo[3] = {l[0] >> 1, l[1] >> 1, l[2] >> 1};
+        const int c0 = lat2node08[C.lat_off + lo[0] + (int64_t)Lx * (lo[1] + (int64_t)Ly * lo[2])];
+        int c1 = -1;
+        if (odd >= 0) {
+            lo[odd] += 1;
+            c1 = lat2node08[C.lat_off + lo[0] + (int64_t)Lx * (lo[1] + (int64_t)Ly * lo[2])];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            float v = 0.f;
+            if (odd < 0) {
+                if (c0 >= 0) v = z08[((int64_t)c0 * 3 + d) * R + lc];
+            } else {
+                if (c0 >= 0) v += 0.5f * z08[((int64_t)c0 * 3 + d) * R + lc];
+                if (c1 >= 0) v += 0.5f * z08[((int64_t)c1 * 3 + d) * R + lc];
+            }
+            z[(q * 3 + d) * R + lc] += v;
+        }
+    }
+}
+
+// res = b - t on the rows of the pseudo-bricks (t = K x, masked on Dirichlet rows)
+__global__ __launch_bounds__(NT_UPD) void k_sub(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                               const float* __restrict__ b, const float* __restrict__ t,
+                                               float* __restrict__ res) {
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done && done[B.dom]) return;
+    const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
+    for (int64_t i = threadIdx.x; i < cnt; i += NT_UPD) res[base + i] = b[base + i] - t[base + i];
+}
+
 inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 // Grids of fewer than FILL_BLOCKS workgroups leave CUs idle: the matvec then
@@ -1819,6 +1926,24 @@ inline int chunk_split(int n_bricks, int nch) {
         if (nch % s == 0) best = s;
     return best;
 }
+// Launch-shape autotuner.  The split of a brick's load-case chunks over
+// gridDim.y workgroups (1, 2, 3 or 6) changes how the chip is filled, not the
+// arithmetic (every chunk is computed whole by one workgroup), so it can be
+// picked by timing without changing any result.  autotune_splits() times the
+// candidates of each FP32 matvec kernel on each multigrid level of a batch
+// before its first solve and caches the fastest per (kernel, variant, bricks);
+// the launches look the choice up (chunk_split's rule when absent, or with
+// DTC_TUNE=0).  DTC_TUNE_LOG=1 prints each decision.
+enum TuneKind { TK_MV = 0, TK_CHEB = 1, TK_BRES = 2 };
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tune;
+
+int tuned_split(int kind, int variant, int n_bricks, int nch) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(std::make_tuple(kind, variant, n_bricks));
+    return it != g_tune.end() ? it->second : chunk_split(n_bricks, nch);
+}
+
 inline int node_split(int n_bricks) {
     const int want = n_bricks > 0 ? (FILL_BLOCKS + n_bricks - 1) / n_bricks : 1;
     return want < 1 ? 1 : want > 8 ? 8 : want;
@@ -1927,6 +2052,7 @@ int batch_begin(SolverBatch& b, const dtc_domain_desc* d, int64_t n_dom) {
     b.d20.clear();
     b.h_contrast.clear();
     b.n_slices = 0;   // the block PCG slices follow the batch's domains (block_setup)
+    b.block_graph.reset();   // captured with the previous composition's buffers
     return DTC_OK;
 }
 
@@ -2102,6 +2228,44 @@ int build_tables(hipStream_t s, std::vector<BrickInfo>& hb, DevMem& brick, const
 
 }  // namespace
 
+// B operands of k_matvec_mf: B[4 b' + d'][4 b + d] = Khat(3 b + d, 3 b' + d') / kmax in
+// corner-bit order (b = ox + 2 oy + 4 oz), direction 3 a zero pad, as FP16 pairs
+// hi + lo; lane l of column tile t holds column j = l % 16 (n = 16 t + j) and
+// rows 8 (l / 16) .. 8 (l / 16) + 7.  Also which FP32 matvecs use it: none unless
+// DTC_MFMA=1 (then Q = K P with 3 FP16 terms and the smoothing steps with
+// DTC_MG_TERMS = 1 or 3, default 3).
+int mf_table(const double* Khat, SolverBatch& b) {
+    int l_of_b[8];
+    for (int l = 0; l < 8; ++l) l_of_b[h08_off(l, 0) + 2 * h08_off(l, 1) + 4 * h08_off(l, 2)] = l;
+    double kmax = 0.0;
+    for (int i = 0; i < 576; ++i) kmax = std::fmax(kmax, std::fabs(Khat[i]));
+    if (!(kmax > 0.0)) return set_error(DTC_ERR_DEVICE, "element stiffness is zero");
+    std::vector<_Float16> tab((size_t)2 * 2 * 64 * 8);
+    for (int t = 0; t < 2; ++t)
+        for (int q = 0; q < 2; ++q)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) {
+                    const int k = 8 * (l / 16) + e, n = 16 * t + (l % 16);
+                    const int bo = n / 4, dd = n % 4, bi = k / 4, di = k % 4;
+                    const double v = (dd < 3 && di < 3)
+                                         ? Khat[(3 * l_of_b[bo] + dd) + 24 * (3 * l_of_b[bi] + di)] / kmax
+                                         : 0.0;
+                    const _Float16 hi = (_Float16)v;
+                    tab[(((size_t)t * 2 + q) * 64 + l) * 8 + e] = q == 0 ? hi : (_Float16)(v - (double)hi);
+                }
+    DTC_TRY(b.kmf.ensure(tab.size() * sizeof(_Float16)));
+    DTC_HIP(hipMemcpyAsync(b.kmf.p, tab.data(), tab.size() * sizeof(_Float16), hipMemcpyHostToDevice, b.stream));
+    DTC_HIP(hipStreamSynchronize(b.stream));
+    b.kmf_max = (float)kmax;
+    const char* e = std::getenv("DTC_MFMA");
+    const bool on = e && e[0] == '1';   // opt-in: slower than the Walsh kernel on the bench batch (measured)
+    b.mf_cg = on ? 1 : 0;
+    int terms = 3;
+    if (const char* t = std::getenv("DTC_MG_TERMS")) terms = std::atoi(t) == 1 ? 1 : 3;
+    b.mf_smooth = on ? terms : 0;
+    return DTC_OK;
+}
+
 int batch_finalize(SolverBatch& b, int precision) {
     b.n_bricks = (int)b.h_brick.size();
     hipStream_t s = b.stream;
@@ -2144,6 +2308,7 @@ int batch_finalize(SolverBatch& b, int precision) {
     DTC_TRY(element_stiffness(8, 1.0, b.nu, b.a, Khat));
     DTC_TRY(walsh_blocks(Khat, &b.wk64));
     b.kdiag = Khat[0];
+    DTC_TRY(mf_table(Khat, b));
     // element-by-element bound lambda_max(D^-1 K) <= lambda_max(diag(Khat)^-1/2 Khat diag(Khat)^-1/2);
     // the diagonal of the cube element is constant
     b.cheb_lmax = sym_lmax(Khat, 24) / b.kdiag;
@@ -2163,6 +2328,7 @@ int batch_finalize(SolverBatch& b, int precision) {
 
 int batch_build_mg(SolverBatch& b, int max_levels) {
     if (b.micro20) return set_error(DTC_ERR_UNSUPPORTED, "the multigrid preconditioner implements HEX08 micro elements");
+    b.block_graph.reset();   // the levels' buffers are rebuilt
     hipStream_t s = b.stream;
     int maxn = 0;
     for (const DomInfo& D : b.h_dom) maxn = std::max(maxn, std::max(D.nx, std::max(D.ny, D.nz)));
@@ -2327,8 +2493,9 @@ int harvest_events(SolverBatch& b) {
     return DTC_OK;
 }
 
+// split > 0: that gridDim.y (the autotuner's candidates), else the tuned one
 template <typename T, int IN, int OUT>
-int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* part, const int* done) {
+int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* part, const int* done, int split = 0) {
     if (b.n_bricks == 0) return DTC_OK;
     MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(), b.einfo.as<int>(),
              sizeof(T) == 4 ? b.eE32.p : b.eE64.p, done, part};
@@ -2340,6 +2507,17 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
     if (kind >= 0) DTC_TRY(timed_begin(b, kind, &slot));
     if (b.micro20) {
         DTC_TRY(mv20(b, sizeof(T) == 8, IN, OUT, xin, yout, part, done));
+    } else if constexpr (sizeof(T) == 4 && IN == IN_V && (OUT == OUT_MASKED || OUT == OUT_MASKED_DOT)) {
+        if (b.mf_cg) {
+            const dim3 grid(b.n_bricks, split > 0 ? split : tuned_split(TK_MV, 3, b.n_bricks, R / MF_RC));
+            k_matvec_mf<OUT, 3><<<grid, MV_NT, 0, b.stream>>>(a, b.kmf.as<uint4>(), b.kmf_max,
+                                                             reinterpret_cast<const float*>(xin),
+                                                             reinterpret_cast<float*>(yout));
+        } else {
+            const dim3 grid(b.n_bricks, split > 0 ? split : tuned_split(TK_MV, 0, b.n_bricks, R / MvCfg<T>::RC));
+            k_matvec<T, IN, OUT><<<grid, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
+        }
+        DTC_HIP(hipGetLastError());
     } else {
         const dim3 grid(b.n_bricks, chunk_split(b.n_bricks, R / MvCfg<T>::RC));
         k_matvec<T, IN, OUT><<<grid, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
@@ -2361,19 +2539,32 @@ struct MvTopo {
     const void* eE32;
 };
 
+// element products of the smoothing matvecs: 0 Walsh VALU kernel, else MFMA with
+// `terms` FP16 terms (table and normalisation of the batch)
+struct MfUse {
+    int terms;
+    const uint4* tab;
+    float kmax;
+};
+
 // tb: the batch whose fine level this is (its launches are timed), else null
 template <int OUT>
 int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float* x, float* y, const int* done,
                 const float* bv = nullptr, const float* dinv = nullptr, float* dv = nullptr, float c1 = 0.f,
                 float c2 = 0.f, double* partial = nullptr, int cflags = CH_READ_D | CH_WRITE_D,
-                SolverBatch* tb = nullptr) {
+                SolverBatch* tb = nullptr, MfUse mf = MfUse{0, nullptr, 0.f}, int split = 0) {
     if (T.n_bricks == 0) return DTC_OK;
     MvArgs a{T.bricks, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2, cflags};
-    const dim3 grid(T.n_bricks, chunk_split(T.n_bricks, R / MvCfg<float>::RC));
+    const int tk = OUT == OUT_CHEB ? TK_CHEB : TK_BRES;
+    const dim3 grid(T.n_bricks, split > 0 ? split : tuned_split(tk, mf.terms, T.n_bricks, R / MvCfg<float>::RC));
     constexpr int kind = OUT == OUT_CHEB ? DTC_KSTAT_CHEB : OUT == OUT_BRES ? DTC_KSTAT_BRES : -1;
     int slot = -1;
     if (tb && kind >= 0) DTC_TRY(timed_begin(*tb, kind, &slot));
-    k_matvec<float, IN_V, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
+    if (mf.terms == 3 && partial) k_matvec_mf<OUT, 3, true><<<grid, MV_NT, 0, s>>>(a, mf.tab, mf.kmax, x, y);
+    else if (mf.terms == 3) k_matvec_mf<OUT, 3><<<grid, MV_NT, 0, s>>>(a, mf.tab, mf.kmax, x, y);
+    else if (mf.terms == 1 && partial) k_matvec_mf<OUT, 1, true><<<grid, MV_NT, 0, s>>>(a, mf.tab, mf.kmax, x, y);
+    else if (mf.terms == 1) k_matvec_mf<OUT, 1><<<grid, MV_NT, 0, s>>>(a, mf.tab, mf.kmax, x, y);
+    else k_matvec<float, IN_V, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
     DTC_HIP(hipGetLastError());
     if (tb) {
         // x, y, b; Chebyshev: the direction read and written as the flags say
@@ -2417,12 +2608,16 @@ MgParams mg_params() {
 // prolongation and its transpose as restriction.  Every smoothing step after the
 // first is one fused matvec (OUT_CHEB); the residual before restriction is one
 // OUT_BRES matvec.  prz (optional) receives per-brick sums of r . z.
+int pmg_vcycle(SolverBatch& b, const float* r, float* z, double* prz);
+
 int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, double* prz) {
+    if (b.micro20) return pmg_vcycle(b, r, z, prz);
     hipStream_t s = b.stream;
     const int L = (int)b.mg.size();
-    const int* done = b.done.as<int>();
+    const int* done = b.done_ext ? b.done_ext : b.done.as<int>();
     static const MgParams P = mg_params();
     const int deg = b.mg_degree > 0 ? b.mg_degree : 2;
+    const MfUse mfu{b.mf_smooth, b.kmf.as<uint4>(), b.kmf_max};
     auto view = [&](int l) {
         LvView v;
         if (l == 0) {
@@ -2471,7 +2666,7 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
             } else {
                 const int fl = (k == 0 ? 0 : (zero && k == 1) ? CH_D_FROM_X : CH_READ_D) | (k < m - 1 ? CH_WRITE_D : 0);
                 DTC_TRY(launch_mv32<OUT_CHEB>(v.T, s, wk, v.buf[cur[l]], v.buf[1 - cur[l]], done, v.b, v.dinv, v.d,
-                                              c1, c2, dk, fl, l == 0 ? &b : nullptr));
+                                              c1, c2, dk, fl, l == 0 ? &b : nullptr, mfu));
                 cur[l] = 1 - cur[l];
             }
         }
@@ -2487,7 +2682,7 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
         if (c.T.n_bricks == 0 || f.T.n_bricks == 0) continue;
         float* res = f.buf[1 - cur[l]];
         DTC_TRY(launch_mv32<OUT_BRES>(f.T, s, wk, f.buf[cur[l]], res, done, f.b, nullptr, nullptr, 0.f, 0.f, nullptr,
-                                      0, l == 0 ? &b : nullptr));
+                                      0, l == 0 ? &b : nullptr, mfu));
         k_restrict<<<dim3(c.T.n_bricks, node_split(c.T.n_bricks)), 256, 0, s>>>(c.T.bricks, c.T.doms, c.l2n, f.T.doms, f.l2n, res, done, c.b);
         DTC_HIP(hipGetLastError());
     }
@@ -2501,6 +2696,91 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
         }
         DTC_TRY(smooth(l, deg, false, P.ratio, l == 0 ? prz : nullptr));
     }
+    return DTC_OK;
+}
+
+// p-multigrid V-cycle of HEX20 micro domains (FP32): Chebyshev-Jacobi smoothing
+// on the HEX20 operator (degree mg_degree, zero start), the residual restricted
+// to the HEX08 discretisation of the same voxels (P^T), that batch's geometric
+// V-cycle, the correction prolongated (P), and the same Chebyshev polynomial
+// after it (a symmetric preconditioner).
+int pmg_vcycle(SolverBatch& b, const float* r, float* z, double* prz) {
+    if (!b.p08) return set_error(DTC_ERR_DEVICE, "p-multigrid without its HEX08 level");
+    SolverBatch& c = *b.p08;
+    hipStream_t s = b.stream;
+    const int* done = b.done.as<int>();
+    static const MgParams P = mg_params();
+    const int deg = b.mg_degree > 0 ? b.mg_degree : 2;
+    const BrickInfo* bl = b.brick.as<BrickInfo>();
+    float* t = b.t32.as<float>();
+    float* d = b.d32.as<float>();
+    const double hi = b.cheb_lmax * P.lscale, lo = hi / P.ratio;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    auto smooth = [&](bool zero, double* dot) -> int {
+        double rho = 1.0 / sigma;
+        for (int k = 0; k < deg; ++k) {
+            float c1 = 0.f, c2 = (float)(1.0 / theta);
+            if (k > 0) {
+                const double rn = 1.0 / (2.0 * sigma - rho);
+                c1 = (float)(rn * rho);
+                c2 = (float)(2.0 * rn / delta);
+                rho = rn;
+            }
+            double* dk = k == deg - 1 ? dot : nullptr;
+            if (zero && k == 0) {
+                k_cheb<<<b.n_bricks, NT_UPD, 0, s>>>(bl, done, r, nullptr, b.dinv32.as<float>(), d, z, c1, c2, 1, dk);
+            } else {
+                DTC_TRY(mv20(b, false, IN_V, OUT_MASKED, z, t, nullptr, done));
+                k_cheb<<<b.n_bricks, NT_UPD, 0, s>>>(bl, done, r, t, b.dinv32.as<float>(), d, z, c1, c2, 0, dk);
+            }
+            DTC_HIP(hipGetLastError());
+        }
+        return DTC_OK;
+    };
+    DTC_TRY(smooth(true, nullptr));
+    // residual, restricted to the HEX08 nodes of each domain group
+    DTC_TRY(mv20(b, false, IN_V, OUT_MASKED, z, t, nullptr, done));
+    k_sub<<<b.n_bricks, NT_UPD, 0, s>>>(bl, done, r, t, t);
+    DTC_HIP(hipGetLastError());
+    float* r08 = c.r32.as<float>();
+    float* z08 = c.z32.as<float>();
+    for (int g = 0; g < b.n_dom && g < c.n_dom; ++g) {
+        const DomInfo& D08 = c.h_dom[g];
+        if (b.h_status[g] != DTC_OK || c.h_status[g] != DTC_OK) continue;
+        const int64_t n = (int64_t)(D08.nx + 1) * (D08.ny + 1) * (D08.nz + 1) * R;
+        k_pmg_restrict<<<nb(n, 256), 256, 0, s>>>(b.h_d20dev[g], D08, c.lat2node.as<int>(), done, g, t, r08);
+    }
+    DTC_HIP(hipGetLastError());
+    c.done_ext = done;
+    DTC_TRY(mg_vcycle(c, cast_wk<float>(c.wk64), r08, z08, nullptr));
+    k_pmg_prolong<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.dom.as<DomInfo>(), b.d20dev.as<Dom20Dev>(), c.dom.as<DomInfo>(),
+                                                c.lat2node.as<int>(), done, z08, z);
+    DTC_HIP(hipGetLastError());
+    return smooth(false, prz);
+}
+
+// p-multigrid set-up of a HEX20 batch over its HEX08 twin c (same domain groups,
+// finalised): the twin's geometric levels, the fine Chebyshev bound and vectors
+int build_pmg(SolverBatch& b, SolverBatch& c) {
+    if (!b.micro20 || c.micro20) return set_error(DTC_ERR_BAD_ARGUMENT, "p-multigrid needs a HEX20 batch and its HEX08 twin");
+    if (c.n_dom != b.n_dom) return set_error(DTC_ERR_DEVICE, "p-multigrid: %d HEX08 groups for %d HEX20 groups", c.n_dom,
+                                             b.n_dom);
+    b.block_graph.reset();
+    c.stream = b.stream;
+    c.mg_degree = b.mg_degree;
+    DTC_TRY(batch_build_mg(c, 8));
+    // element-by-element bound of lambda_max(D^-1 K) for HEX20:
+    // lambda_max(diag(Khat)^-1/2 Khat diag(Khat)^-1/2)
+    std::vector<double> S(3600);
+    for (int i = 0; i < 60; ++i)
+        for (int j = 0; j < 60; ++j)
+            S[i + 60 * j] = b.khat20[i + 60 * j] / std::sqrt(b.khat20[i + 60 * i] * b.khat20[j + 60 * j]);
+    b.cheb_lmax = sym_lmax(S.data(), 60);
+    const size_t v0 = (size_t)(b.n_nodes > 0 ? b.n_nodes : 1) * 3 * R * 4;
+    DTC_TRY(b.z32.ensure(v0));
+    DTC_TRY(b.t32.ensure(v0));
+    DTC_TRY(b.d32.ensure(v0));
+    b.p08 = &c;
     return DTC_OK;
 }
 
@@ -2649,16 +2929,105 @@ int block_iteration(SolverBatch& b, const WK<float>& wk, float* x, float* r, flo
     return block_orth(b, q, p);
 }
 
-// block PCG iterations until every domain is done or max_steps iterations ran
+// *out = 1 when every domain's flag is set (one workgroup, fixed order)
+__global__ __launch_bounds__(256) void k_all_set(const int* __restrict__ flags, int n, int* __restrict__ out) {
+    __shared__ int s_any[256];
+    int miss = 0;
+    for (int d = threadIdx.x; d < n; d += 256) miss |= flags[d] == 0;
+    s_any[threadIdx.x] = miss;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_any[threadIdx.x] |= s_any[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s_any[0] ? 0 : 1;
+}
+
+// block PCG iterations until every domain is done or max_steps iterations ran.
+//
+// Without kernel timing (the default), GROUP iterations are captured once per
+// solve into a HIP graph together with a kernel that writes "every domain done"
+// to a device word; the graph is replayed, and after each replay that word is
+// copied into one of two page-locked host words.  The host decides to stop
+// from the previous replay's word while the current replay already runs, so the
+// GPU never waits for the host; a replay issued after every domain finished is
+// a sequence of early exits (every kernel skips done domains), which leaves the
+// results unchanged.  DTC_GRAPH=0 (or kernel timing) launches the iterations one
+// by one with a synchronous check every GROUP iterations.
 int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p, float* q, float* z,
               int64_t max_iters, int64_t max_steps) {
-    const int CHECK = 4;
-    for (int64_t steps = 0; steps < max_steps; steps += CHECK) {
-        bool fin = false;
-        DTC_TRY(all_done(b, b.done, &fin));
-        if (fin) return DTC_OK;
-        for (int k = 0; k < CHECK; ++k) DTC_TRY(block_iteration(b, wk, x, r, p, q, z, max_iters));
+    constexpr int GROUP = 4;
+    const char* ge = std::getenv("DTC_GRAPH");
+    const bool use_graph = !(ge && ge[0] == '0') && !b.timing;
+    bool fin = false;
+    DTC_TRY(all_done(b, b.done, &fin));
+    if (fin) return DTC_OK;
+    if (!use_graph) {
+        for (int64_t steps = 0; steps < max_steps; steps += GROUP) {
+            if (steps > 0) {
+                DTC_TRY(all_done(b, b.done, &fin));
+                if (fin) return DTC_OK;
+            }
+            for (int k = 0; k < GROUP; ++k) DTC_TRY(block_iteration(b, wk, x, r, p, q, z, max_iters));
+        }
+        return DTC_OK;
     }
+    hipStream_t s = b.stream;
+    const void* key[6] = {x, r, p, q, z, b.done.p};
+    bool same = b.block_graph.e && b.graph_iters == max_iters && b.graph_bricks == b.n_bricks;
+    for (int k = 0; k < 6; ++k) same = same && b.graph_key[k] == key[k];
+    if (!same) {
+        b.block_graph.reset();
+        DTC_TRY(b.all_done_dev.ensure(4));
+        if (!b.all_done_host.p) DTC_HIP(hipHostMalloc((void**)&b.all_done_host.p, 2 * sizeof(int), hipHostMallocDefault));
+        hipGraph_t g = nullptr;
+        DTC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = DTC_OK;
+        for (int k = 0; k < GROUP && rc == DTC_OK; ++k) rc = block_iteration(b, wk, x, r, p, q, z, max_iters);
+        if (rc == DTC_OK) k_all_set<<<1, 256, 0, s>>>(b.done.as<int>(), b.n_dom, b.all_done_dev.as<int>());
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        DTC_TRY(rc);
+        if (ec != hipSuccess) return set_error(DTC_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&b.block_graph.e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            b.block_graph.e = nullptr;
+            return set_error(DTC_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        }
+        for (int k = 0; k < 6; ++k) b.graph_key[k] = key[k];
+        b.graph_iters = max_iters;
+        b.graph_bricks = b.n_bricks;
+    }
+    // launch n copies the summary to host word n % 2 and records event n % 2
+    hipEvent_t ev[2];
+    for (hipEvent_t& e : ev) DTC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    int rc = DTC_OK;
+    int64_t launched = 0;
+    auto launch = [&](int64_t n) -> int {
+        DTC_HIP(hipGraphLaunch(b.block_graph.e, s));
+        DTC_HIP(hipMemcpyAsync(b.all_done_host.p + (n & 1), b.all_done_dev.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        DTC_HIP(hipEventRecord(ev[n & 1], s));
+        ++b.graph_launches;
+        return DTC_OK;
+    };
+    rc = launch(0);
+    launched = 1;
+    for (int64_t steps = GROUP; rc == DTC_OK && steps < max_steps; steps += GROUP) {
+        rc = launch(launched);
+        if (rc != DTC_OK) break;
+        ++launched;
+        // the replay before this one has finished: its summary decides
+        const hipError_t e = hipEventSynchronize(ev[(launched - 2) & 1]);
+        if (e != hipSuccess) {
+            rc = set_error(DTC_ERR_DEVICE, "hipEventSynchronize: %s", hipGetErrorString(e));
+            break;
+        }
+        if (b.all_done_host.p[(launched - 2) & 1]) break;
+    }
+    const hipError_t es = hipStreamSynchronize(s);
+    for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
+    DTC_TRY(rc);
+    if (es != hipSuccess) return set_error(DTC_ERR_DEVICE, "hipStreamSynchronize: %s", hipGetErrorString(es));
     return DTC_OK;
 }
 
@@ -2751,13 +3120,95 @@ int apply_cap(SolverBatch& b, int64_t cap, std::vector<int64_t>& nouter, bool* a
 
 }  // namespace
 
+// Times every candidate split of the FP32 matvec kernels the mixed multigrid solve
+// launches (Q = K P on the fine level; the Chebyshev step and the restriction
+// residual on every level) on this batch's own tables, and caches the fastest
+// (see tuned_split).  The timed launches read the level's work vectors and write
+// its scratch vector, before the solve starts, with no done flags.
+int autotune_splits(SolverBatch& b, const WK<float>& wk) {
+    const char* te = std::getenv("DTC_TUNE");
+    if ((te && te[0] == '0') || b.n_bricks == 0 || b.micro20) return DTC_OK;
+    const char* le = std::getenv("DTC_TUNE_LOG");
+    const bool log = le && le[0] == '1';
+    hipStream_t s = b.stream;
+    hipEvent_t e0, e1;
+    DTC_HIP(hipEventCreate(&e0));
+    DTC_HIP(hipEventCreate(&e1));
+    int rc = DTC_OK;
+    auto measure = [&](int kind, int variant, int nbr, const char* what, int level,
+                       const std::function<int(int)>& launch) {
+        if (rc != DTC_OK || nbr == 0) return;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            if (g_tune.count(std::make_tuple(kind, variant, nbr))) return;
+        }
+        const int cand[4] = {1, 2, 3, 6};
+        int best = chunk_split(nbr, R / MF_RC);
+        float best_ms = 1e30f;
+        for (int c : cand) {
+            if ((rc = launch(c)) != DTC_OK) return;   // warm-up
+            if (hipEventRecord(e0, s) != hipSuccess) { rc = set_error(DTC_ERR_DEVICE, "hipEventRecord"); return; }
+            for (int k = 0; k < 3 && rc == DTC_OK; ++k) rc = launch(c);
+            if (rc != DTC_OK) return;
+            float ms = 0.f;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = set_error(DTC_ERR_DEVICE, "autotune timing");
+                return;
+            }
+            if (ms < best_ms) { best_ms = ms; best = c; }
+        }
+        if (log)
+            std::fprintf(stderr, "dtc autotune: %s level %d, %d bricks: gridDim.y %d (%.1f us)\n", what, level, nbr,
+                         best, 1e3f * best_ms / 3.f);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[std::make_tuple(kind, variant, nbr)] = best;
+    };
+    float* p = b.p32.as<float>();
+    float* q = b.q32.as<float>();
+    measure(TK_MV, b.mf_cg ? 3 : 0, b.n_bricks, "Q = K P", 0, [&](int c) {
+        return launch_mv<float, IN_V, OUT_MASKED>(b, wk, p, q, nullptr, nullptr, c);
+    });
+    const MfUse mfu{b.mf_smooth, b.kmf.as<uint4>(), b.kmf_max};
+    for (int l = 0; l <= (int)b.mg.size() && rc == DTC_OK; ++l) {
+        MvTopo T;
+        float *x, *bb, *y, *d;
+        const float* dinv;
+        if (l == 0) {
+            T = {b.n_bricks, b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(),
+                 b.einfo.as<int>(), b.eE32.p};
+            x = b.z32.as<float>(); bb = b.r32.as<float>(); y = b.t32.as<float>(); d = b.d32.as<float>();
+            dinv = b.dinv32.as<float>();
+        } else {
+            Level& C = b.mg[l - 1];
+            T = {C.n_bricks, C.brick.as<BrickInfo>(), C.dom.as<DomInfo>(), C.halo.as<int>(), C.nmask.as<unsigned char>(),
+                 C.einfo.as<int>(), C.eE32.p};
+            x = C.x.as<float>(); bb = C.b.as<float>(); y = C.t.as<float>(); d = C.d.as<float>();
+            dinv = C.dinv32.as<float>();
+        }
+        if (!x || !bb || !y) continue;
+        measure(TK_CHEB, mfu.terms, T.n_bricks, "Chebyshev step", l, [&](int c) {
+            return launch_mv32<OUT_CHEB>(T, s, wk, x, y, nullptr, bb, dinv, d, 0.f, 1.f, nullptr, 0, nullptr, mfu, c);
+        });
+        measure(TK_BRES, mfu.terms, T.n_bricks, "restriction residual", l, [&](int c) {
+            return launch_mv32<OUT_BRES>(T, s, wk, x, y, nullptr, bb, nullptr, nullptr, 0.f, 0.f, nullptr, 0, nullptr,
+                                         mfu, c);
+        });
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int batch_build_pmg(SolverBatch& b, SolverBatch& c) { return build_pmg(b, c); }
+
 int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>& iters,
                 std::vector<int64_t>& outer, std::vector<double>& relres) {
     const int D = b.n_dom;
     // with the multigrid, the 24 load cases of a domain iterate as one block
     // (DTC_BLOCK_PCG=0: one PCG per load case)
     const char* be = std::getenv("DTC_BLOCK_PCG");
-    const bool block = b.precision == DTC_FP32_REFINE && b.precond == DTC_PC_MG && !b.micro20 &&
+    const bool block = b.precision == DTC_FP32_REFINE && b.precond == DTC_PC_MG && (!b.micro20 || b.p08) &&
                        !(be && be[0] == '0');
     // inner_tol outside (0, 1): automatic, per domain.  The block restarts at every
     // FP64 residual replacement, so it takes its FP32 phases as deep as FP32
@@ -2803,6 +3254,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
     DTC_HIP(hipGetLastError());
     for (KernelStat& k : b.kstat) k = KernelStat{};
     b.reactions_valid = false;
+    b.graph_launches = 0;
     if (b.timing) {
         // per domain group: vector bytes (FP32), moduli (FP32), Jacobi scaling (FP32), vector bytes (FP64)
         std::vector<double> db((size_t)D * 4, 0.0);
@@ -2849,7 +3301,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
         // (or meets the final target, or after REPLACE iterations), r32 is
         // replaced by the true FP64 residual and the search direction p32 is
         // kept.
-        const bool mg = b.precond == DTC_PC_MG && !b.micro20;
+        const bool mg = b.precond == DTC_PC_MG && (!b.micro20 || b.p08);
         Progress progress;   // domains that stop converging are given up (the caller repeats them in FP64)
         const int64_t REPLACE = mg ? 100 : 1000;
         float* x32 = b.x32.as<float>();
@@ -2868,6 +3320,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
         float* bq = b.q32.as<float>();
         float* bz = z32;
         if (block) DTC_TRY(block_setup(b));
+        if (mg) DTC_TRY(autotune_splits(b, wk32));
         bool first = true;
         for (int it = 0; it < 100000; ++it) {
             bool left = false;

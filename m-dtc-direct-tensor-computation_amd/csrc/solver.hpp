@@ -29,6 +29,8 @@ struct Dom20 {
     DevMem adj_off, adj;           // node -> sorted (element << 5 | local node)
     DevMem bcf;                    // uint8 per node: Dirichlet node
     DevMem scan_tmp;
+    DevMem lat20;                  // int [(2nx+1)(2ny+1)(2nz+1)]: local node at a half-voxel lattice point, -1 none
+    int Lx = 0, Ly = 0, Lz = 0;    // half-voxel lattice extents
     int64_t node_begin = 0, n_nodes = 0;
 };
 struct Dom20Dev {
@@ -38,8 +40,34 @@ struct Dom20Dev {
     const int* adj_off;
     const int* adj;
     const unsigned char* bcf;
+    const int* lat20;
     int64_t node_begin;
     int n_nodes;
+    int Lx, Ly, Lz;
+};
+
+// An instantiated HIP graph (destroyed with its owner).
+struct GraphExec {
+    hipGraphExec_t e = nullptr;
+    GraphExec() = default;
+    GraphExec(const GraphExec&) = delete;
+    GraphExec& operator=(const GraphExec&) = delete;
+    ~GraphExec() { reset(); }
+    void reset() {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+};
+
+// Page-locked host words the device writes (hipHostMalloc, freed with the owner).
+struct HostFlags {
+    int* p = nullptr;
+    HostFlags() = default;
+    HostFlags(const HostFlags&) = delete;
+    HostFlags& operator=(const HostFlags&) = delete;
+    ~HostFlags() {
+        if (p) (void)hipHostFree(p);
+    }
 };
 
 // One batch of domains resident on the device: topology, vectors, scalars.
@@ -95,11 +123,24 @@ struct SolverBatch {
     double cheb_lmax = 0.0;        // element-by-element bound of lambda_max(D^-1 K)
     // HEX20 micro elements: node-gather operator over pseudo-bricks of mesh nodes
     bool micro20 = false;
+    // p-multigrid of HEX20 micro domains: the HEX08 discretisation of the same
+    // voxels (a batch of its own, same domain groups, its geometric V-cycle) as
+    // the coarse space; the fine level is smoothed with Chebyshev-Jacobi on the
+    // interval [cheb_lmax / ratio, cheb_lmax] of D^-1 K (HEX20 element bound)
+    SolverBatch* p08 = nullptr;
+    const int* done_ext = nullptr;  // done flags to use instead of `done` (the coarse batch of a p-multigrid)
     std::vector<Dom20> d20;        // per domain group (empty entries for failed groups)
     DevMem d20dev;                 // Dom20Dev[]
+    std::vector<Dom20Dev> h_d20dev;   // host copy
     std::vector<double> khat20;    // E = 1 element stiffness, 60 x 60
     DevMem k20_64, k20_32;
     WalshK wk64{};
+    // element products on the matrix cores (FP32 matvecs): B operands of the
+    // normalised element stiffness, its normalisation, and which kernels use them
+    DevMem kmf;                    // uint4 [tile 2][term 2][lane 64]
+    float kmf_max = 0.f;
+    int mf_cg = 0;                 // Q = K P (and q = K p): 0 Walsh VALU kernel, 1 MFMA (3 FP16 terms)
+    int mf_smooth = 0;             // smoothing and restriction residual: 0 Walsh, 1 or 3 MFMA terms
     double kdiag = 0.0;            // diagonal entry of the E = 1 element stiffness
     double nu = 0.3, a = 1.0;
     std::vector<double> h_contrast;   // per domain group: modulus contrast (max / min) over its solid voxels
@@ -119,6 +160,16 @@ struct SolverBatch {
     int n_slots = 0;
     KernelStat kstat[DTC_KSTAT_COUNT];
     bool timing = false;
+    // the block PCG's iterations captured as one HIP graph (replayed until every
+    // domain is done), the arguments it was captured with, and the device word /
+    // page-locked host words of its "all domains done" summary
+    GraphExec block_graph;
+    const void* graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int64_t graph_iters = -1;
+    int graph_bricks = -1;
+    DevMem all_done_dev;           // int: 1 when every domain's done flag is set
+    HostFlags all_done_host;       // int [2]: copies of all_done_dev, one per launch parity
+    int graph_launches = 0;        // graph launches of the last solve (statistics)
 };
 
 // Walsh-basis block coefficients of the E = 1 HEX08 element stiffness.
@@ -140,6 +191,9 @@ int batch_add_empty(SolverBatch& b, const dtc_domain_desc& d, int status, int lc
 int batch_finalize(SolverBatch& b, int precision);
 // Build the coarse levels of the multigrid preconditioner (after batch_finalize).
 int batch_build_mg(SolverBatch& b, int max_levels);
+// p-multigrid of a HEX20 micro batch b over c, the HEX08 discretisation of the same
+// voxels and domain groups (both finalised): builds c's levels and links c to b.
+int batch_build_pmg(SolverBatch& b, SolverBatch& c);
 // Full solve: returns per-domain status in b.h_status; fills iteration counts.
 int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& iters,
                 std::vector<int64_t>& outer, std::vector<double>& relres);

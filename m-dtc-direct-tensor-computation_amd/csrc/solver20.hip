@@ -68,6 +68,15 @@ __global__ void k20_adj_fill(const int* __restrict__ elems, int64_t ne, const in
     adj[off[q] + atomicAdd(&cursor[q], 1)] = (e << 5) | a;
 }
 
+// half-voxel lattice point -> local kept node id (-1: no kept node)
+__global__ void k20_latmap(const int* __restrict__ numLat, const int* __restrict__ keepN, const int* __restrict__ cref,
+                           int64_t nl, int* __restrict__ lat20) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const int num = numLat[i];
+    lat20[i] = (num > 0 && keepN[num - 1]) ? cref[num - 1] : -1;
+}
+
 // sort each node's (element, local node) list: fixed summation order
 __global__ void k20_adj_sort(const int* __restrict__ off, int64_t nn, int* __restrict__ adj) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -334,6 +343,12 @@ int batch_add_domain20(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& 
     k20_adj_fill<<<nb(ne * 20, 256), 256, 0, s>>>(G.elems.as<int>(), ne, G.adj_off.as<int>(), cursor.as<int>(),
                                                   G.adj.as<int>());
     k20_adj_sort<<<nb(nn, 256), 256, 0, s>>>(G.adj_off.as<int>(), nn, G.adj.as<int>());
+    // lattice map for the p-multigrid transfers
+    G.Lx = m.Lx; G.Ly = m.Ly; G.Lz = m.Lz;
+    const int64_t nl = (int64_t)m.Lx * m.Ly * m.Lz;
+    DTC_TRY(G.lat20.alloc((size_t)nl * 4));
+    k20_latmap<<<nb(nl, 256), 256, 0, s>>>(m.numLat.as<int>(), m.keepN.as<int>(), m.cref.as<int>(), nl,
+                                           G.lat20.as<int>());
     DTC_HIP(hipGetLastError());
     // pseudo-bricks of PB consecutive nodes
     D.brick_begin = (int)b.h_brick.size();
@@ -372,8 +387,10 @@ int finalize20(SolverBatch& b) {
     for (int i = 0; i < b.n_dom; ++i) {
         const Dom20& G = b.d20[i];
         h[i] = Dom20Dev{G.coords.as<double>(), G.elems.as<int>(), G.emod.as<double>(), G.adj_off.as<int>(),
-                        G.adj.as<int>(), G.bcf.as<unsigned char>(), G.node_begin, (int)G.n_nodes};
+                        G.adj.as<int>(), G.bcf.as<unsigned char>(), G.lat20.as<int>(), G.node_begin, (int)G.n_nodes,
+                        G.Lx, G.Ly, G.Lz};
     }
+    b.h_d20dev = h;
     DTC_TRY(b.d20dev.ensure(std::max<size_t>(h.size(), 1) * sizeof(Dom20Dev)));
     if (!h.empty()) DTC_HIP(hipMemcpyAsync(b.d20dev.p, h.data(), h.size() * sizeof(Dom20Dev), hipMemcpyHostToDevice, s));
     std::vector<float> k32(3600);

@@ -156,7 +156,10 @@ program dtc_crawl_tensors
         write(ucovo(t), '(36(",",ES24.16))') ((mat(kk, ll), ll = 1, 6), kk = 1, 6)
         if (t == 1) then
            write(unum, '(I0)', advance="no") dom(slot)
-           write(unum, '(*(",",ES24.16))') (knum(int(nlc, i8) * nlc * (slot - 1) + c), c = 1, nlc * nlc)
+           do c = 1, nlc * nlc
+              write(unum, '(",",ES24.16)', advance="no") knum(int(nlc, i8) * nlc * (slot - 1) + c)
+           end do
+           write(unum, '(A)') ""
            nrows = nrows + 1
         end if
      end do

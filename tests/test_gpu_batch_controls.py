@@ -122,3 +122,54 @@ def test_kernel_timing_statistics():
         assert lo * (1 - 1e-12) <= s["bytes"] <= hi * (1 + 1e-12), (k, s, lo, hi)
     # the block PCG runs one Q = K P per iteration, the V-cycle three fused steps
     assert st[dtc.DTC_KSTAT_CHEB]["launches"] >= 2 * st[dtc.DTC_KSTAT_MV]["launches"]
+
+
+def test_graph_replay_equals_launch_by_launch(monkeypatch):
+    """The block PCG replayed as a captured HIP graph with the stop decided one
+    replay late (the default) gives, bit for bit, the results of launching every
+    kernel from the host with a synchronous check (DTC_GRAPH=0): a replay after
+    every domain finished changes nothing, and each domain's iteration count is
+    its own."""
+    dtc = _dtc()
+    doms = [_binary(dtc, 16 + 4 * i, 80 + i) for i in range(3)] + [_grey(dtc, 20, 83)]
+    kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+    monkeypatch.setenv("DTC_GRAPH", "0")
+    with dtc.Batch(doms, **kw) as b:
+        r0 = b.run()
+        g0 = b.grad(1, np.eye(6))
+    monkeypatch.delenv("DTC_GRAPH")
+    with dtc.Batch(doms, **kw) as b:
+        r1 = b.run()
+        r2 = b.run()                                  # captured again for the rebuilt multigrid levels
+        g1 = b.grad(1, np.eye(6))
+    for a, c, e in zip(r0, r1, r2):
+        assert a["status"] == c["status"] == e["status"] == 0
+        assert a["iterations"] == c["iterations"] == e["iterations"]
+        assert a["outer_iterations"] == c["outer_iterations"]
+        for k in ("C", "K_macro", "ff"):
+            assert np.array_equal(a[k], c[k]) and np.array_equal(a[k], e[k]), k
+    assert np.array_equal(g0, g1)
+
+
+@pytest.mark.parametrize("terms", ["1", "3"])
+def test_mfma_element_products_match_the_walsh_kernel(monkeypatch, terms):
+    """The FP32 matvecs with the element products on the matrix cores (FP16 pairs,
+    DTC_MFMA=1) and the Walsh-basis VALU kernel (the default) solve to the same
+    tensors, to what a relative residual of 1e-8 fixes, binary and greyscale, with
+    1 or 3 FP16 terms in the multigrid smoother.  With 3 terms the iteration counts
+    stay close; one FP16 term (2^-11) in the smoother costs iterations (measured
+    35 -> 49 on the first domain, 101 -> 296 on the bench batch), which is why the
+    Walsh kernel stays the default."""
+    dtc = _dtc()
+    doms = [_binary(dtc, 24, 90), _binary(dtc, 30, 91), _grey(dtc, 20, 92)]
+    kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+    monkeypatch.setenv("DTC_MFMA", "0")
+    rw = dtc.solve_batch(doms, **kw)
+    monkeypatch.setenv("DTC_MFMA", "1")
+    monkeypatch.setenv("DTC_MG_TERMS", terms)
+    rm = dtc.solve_batch(doms, **kw)
+    for a, c in zip(rw, rm):
+        assert a["status"] == c["status"] == 0 and c["relres"] <= 1e-8
+        assert _rel(c["C"], a["C"]) < 1e-7 and _rel(c["K_macro"], a["K_macro"]) < 1e-7
+        if terms == "3":
+            assert c["iterations"] <= 1.25 * a["iterations"] + 4, (a["iterations"], c["iterations"])
