@@ -183,6 +183,67 @@ def config4(args, log):
                       "data": "synthetic", "dtype": "f32+f64"}))
 
 
+def config5(args, log):
+    """Configuration 5: 128^3 greyscale domain (HEX08 micro + macro, 24 load cases),
+    loss = ||C - C_target||_F^2 (C_target a seeded orthotropic SPD tensor); the
+    hand-written sensitivity (no second solve) against central differences of full
+    solves at relative step 1e-4 on `--fd-voxels` seeded voxels among the stiffest
+    tenth (the perturbed domains solved as one batch to rel-res 1e-11), and the
+    gradient of the 1e-8 solve against the one of the FP64-refined 1e-11 solve."""
+    import dtc_amd
+    from dtc_amd import synth
+    n = args.n if args.n != 64 else 128
+    rng = np.random.default_rng(55)
+    C_t = synth.orthotropic_C(rng)
+    grey, modulus = synth.greyscale(n, seed=5, bvtv=0.20)
+    dom = lambda m: dtc_amd.Domain(grey, llimit=1, delta=(0.01,) * 3, modulus=m, E=10000.0)
+    loss = lambda C: float(np.sum((C - C_t) ** 2))
+    out = {}
+    for tol in (1e-8, 1e-11):
+        with dtc_amd.Batch([dom(modulus)], precision=dtc_amd.DTC_FP32_REFINE, rel_tol=tol, rotation=False,
+                           precond=dtc_amd.DTC_PC_MG, max_iter=args.max_iter) as b:
+            _sync()
+            t0 = time.perf_counter()
+            r = b.run()[0]
+            _sync()
+            t_f = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            g = b.grad(0, 2.0 * (r["C"] - C_t))
+            _sync()
+            t_g = time.perf_counter() - t0
+        out[tol] = (r, g, t_f, t_g)
+        log(f"config 5 at rel-res {tol:g}: {r['iterations']} block iterations, forward {t_f:.2f} s, gradient "
+            f"{t_g:.3f} s")
+    g8, g11 = out[1e-8][1], out[1e-11][1]
+    gerr = float(np.abs(g8 - g11).max() / np.abs(g11).max())
+    solid = np.argwhere(grey.ravel() > 0).ravel()
+    stiff = solid[modulus.ravel()[solid] >= np.quantile(modulus.ravel()[solid], 0.9)]
+    vox = rng.choice(stiff, args.fd_voxels, replace=False)
+    doms, hs = [], []
+    for k in vox:
+        h = 1e-4 * modulus.ravel()[k]
+        hs.append(h)
+        for sgn in (1, -1):
+            m2 = modulus.copy().ravel()
+            m2[k] += sgn * h
+            doms.append(dom(m2.reshape(grey.shape)))
+    t0 = time.perf_counter()
+    rf = dtc_amd.solve_batch(doms, precision=dtc_amd.DTC_FP32_REFINE, rel_tol=1e-11, rotation=False,
+                             precond=dtc_amd.DTC_PC_MG, max_iter=args.max_iter)
+    t_fd = time.perf_counter() - t0
+    fd = np.array([(loss(rf[2 * i]["C"]) - loss(rf[2 * i + 1]["C"])) / (2 * hs[i]) for i in range(len(vox))])
+    gk = g11.ravel()[vox]
+    fd_err = float(np.abs(fd - gk).max() / np.abs(g11).max())
+    log(f"config 5: {len(vox)} voxels by central differences ({2 * len(vox)} solves, {t_fd:.1f} s): max |fd - g| / "
+        f"max |g| = {fd_err:.2e}; gradient 1e-8 vs 1e-11: {gerr:.2e}")
+    r8, _, tf8, tg8 = out[1e-8]
+    print(json.dumps({"metric": "config 5: 128^3 greyscale gradient of ||C - C_target||_F^2", "n_gpus": 1,
+                      "forward_s": round(tf8, 3), "gradient_s": round(tg8, 4), "grad_over_forward": round(tg8 / tf8, 4),
+                      "iterations": r8["iterations"], "fd_voxels": len(vox), "fd_rel_err": fd_err,
+                      "grad_1e-8_vs_1e-11": gerr, "fd_batch_s": round(t_fd, 2), "data": "synthetic",
+                      "dtype": "f32+f64"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +268,9 @@ def main():
                          "anisotropic frames; rotation search on a 5 degree grid)")
     ap.add_argument("--config4", action="store_true",
                     help="instead: configuration 4 (96^3 greyscale, HEX20 micro + HEX20 macro, p-multigrid)")
+    ap.add_argument("--config5", action="store_true",
+                    help="instead: configuration 5 (128^3 greyscale gradient, central differences, 1e-11 reference)")
+    ap.add_argument("--fd-voxels", type=int, default=8, help="config 5: voxels checked by central differences")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     if args.gpus != 1:
@@ -220,6 +284,9 @@ def main():
         return
     if args.config4:
         config4(args, log)
+        return
+    if args.config5:
+        config5(args, log)
         return
     doms = make_domains(args.batch, args.n, 2)
     log(f"batch of {len(doms)} domains of {args.n}^3, BV/TV " +

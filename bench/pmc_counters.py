@@ -56,15 +56,22 @@ def main() -> None:
         mine = [r for r in rows if kname in r[0].replace("(anonymous namespace)::", "")]
         if not mine:
             continue
-        if all(r[1] is not None for r in mine):
-            g = max(r[1] for r in mine)
-            mine = [r for r in mine if r[1] == g]
-        per = {}
-        for _, _, disp, cn, v in mine:
-            per.setdefault(cn, {}).setdefault(disp, 0.0)
-            per[cn][disp] += float(v)        # summed over the counter's instances per dispatch
-        res[key] = {"kernel": label, "dispatches": max(len(d) for d in per.values()),
-                    **{cn: sum(d.values()) / len(d) for cn, d in sorted(per.items())}}
+        variants = {}
+        for r in mine:
+            variants.setdefault(r[0], []).append(r)
+        entry = {"kernel": label, "variants": {}}
+        for name, rs in variants.items():
+            if all(r[1] is not None for r in rs):
+                g = max(r[1] for r in rs)
+                rs = [r for r in rs if r[1] == g]
+            per = {}
+            for _, _, disp, cn, v in rs:
+                per.setdefault(cn, {}).setdefault(disp, 0.0)
+                per[cn][disp] += float(v)        # summed over the counter's instances per dispatch
+            short = name.replace("(anonymous namespace)::", "").replace("dtc::", "").split("(")[0]
+            entry["variants"][short] = {"dispatches": max(len(d) for d in per.values()),
+                                        **{cn: sum(d.values()) / len(d) for cn, d in sorted(per.items())}}
+        res[key] = entry
     out = {"workload": f"bench.py batch {args.batch}, {args.n}^3 trabecular domains, one step, fine-level dispatches",
            "counters": counters, "per_dispatch": res,
            "command": "python bench/pmc_counters.py --counters " + ",".join(counters) +

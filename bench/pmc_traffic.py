@@ -29,12 +29,14 @@ import sys
 import tempfile
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-# bench kernel key -> (kernel name as rocprofv3 writes it, the name bench.py reports)
+# bench kernel key -> (kernel name prefix as rocprofv3 writes it, the name bench.py reports).
+# A fine-level FP32 matvec is one launch of the sparse-brick variant and one of the
+# dense variant (template flags after the prefix): their per-dispatch averages add.
 KERNELS = {
-    "mv": ("k_matvec<float, 0, 3, false>", "k_matvec<float, IN_V, OUT_MASKED>"),
-    "cheb": ("k_matvec<float, 0, 4, false>", "k_matvec<float, IN_V, OUT_CHEB>"),
-    "bres": ("k_matvec<float, 0, 5, false>", "k_matvec<float, IN_V, OUT_BRES>"),
-    "resid": ("k_matvec<double, 1, 1, false>", "k_matvec<double, IN_U, OUT_RESID>"),
+    "mv": ("k_matvec<float, 0, 3,", "k_matvec<float, IN_V, OUT_MASKED>"),
+    "cheb": ("k_matvec<float, 0, 4,", "k_matvec<float, IN_V, OUT_CHEB>"),
+    "bres": ("k_matvec<float, 0, 5,", "k_matvec<float, IN_V, OUT_BRES>"),
+    "resid": ("k_matvec<double, 1, 1,", "k_matvec<double, IN_U, OUT_RESID>"),
 }
 COPY = ("import torch; a = torch.ones(2**28, device='cuda'); b = torch.empty_like(a); torch.cuda.synchronize(); "
         "b.copy_(a); torch.cuda.synchronize(); print(float(b[-1]))")
@@ -55,14 +57,23 @@ def _rows(dbdir: str, counter: str):
 
 
 def _counter(dbdir: str, counter: str, kname: str) -> tuple[float, int]:
-    """(mean value per dispatch, dispatches) over the largest-grid dispatches of kname."""
+    """(mean value per fine-level call, dispatches): for every kernel variant whose
+    name starts with kname, the mean over its largest-grid dispatches; the variants'
+    means are added (one dispatch of each per call)."""
     rows = [r for r in _rows(dbdir, counter) if kname in r[0].replace("(anonymous namespace)::", "")]
     if not rows:
         raise RuntimeError(f"no {counter} samples of {kname} in {dbdir}")
-    if all(r[1] is not None for r in rows):
-        g = max(r[1] for r in rows)
-        rows = [r for r in rows if r[1] == g]
-    return sum(r[2] for r in rows) / len(rows), len(rows)
+    by = {}
+    for r in rows:
+        by.setdefault(r[0], []).append(r)
+    total, n = 0.0, 0
+    for name, rs in by.items():
+        if all(r[1] is not None for r in rs):
+            g = max(r[1] for r in rs)
+            rs = [r for r in rs if r[1] == g]
+        total += sum(r[2] for r in rs) / len(rs)
+        n = max(n, len(rs))
+    return total, n
 
 
 def _pmc(counter: str, out: str, cmd: list[str]) -> str:

@@ -192,6 +192,8 @@ template <typename T> struct MvCfg;
 template <> struct MvCfg<float> { static constexpr int RC = 4; using V = float4; };
 template <> struct MvCfg<double> { static constexpr int RC = 2; using V = double2; };
 constexpr int MV_NT = 512;
+constexpr int MV_NT_SP = 256;     // sparse-brick variant
+constexpr int MV_SLOTS = 384;     // (owned node, element corner) pairs of a sparse brick, at most
 constexpr int NH = HX * HY * HZ, NET = EX * EY * EZ, NO = BX * BY * BZ;
 constexpr int OB_BIAS = 1 + BX + BX * BY;   // owned-layout base of tile element (0,0,0) is -OB_BIAS
 
@@ -214,6 +216,8 @@ __device__ inline void vadd(double2& a, const double2& b) { a.x += b.x; a.y += b
 
 struct MvArgs {
     const BrickInfo* bricks;
+    const int* blist;           // bricks of this launch (blist[boff + blockIdx.x]), or null: brick = blockIdx.x
+    int boff;
     const DomInfo* doms;
     const int* halo;            // [brick][NH]
     const unsigned char* nmask; // [brick][NO]: solid tile elements around each owned node
@@ -256,33 +260,43 @@ constexpr int CH_READ_D = 1, CH_D_FROM_X = 2, CH_WRITE_D = 4;
 // launch bounds: at least 4 waves per SIMD (2 workgroups of 8 waves per CU), which
 // caps the kernel at 128 VGPRs
 // IO32 (T = double, IN_V only): x and y are FP32 vectors, the arithmetic is FP64
-template <typename T, int IN, int OUT, bool IO32 = false>
-__global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
-                                                 T* __restrict__ yout) {
+// SPARSE: the variant for bricks with at most MV_SLOTS (owned node, element corner)
+// pairs: 256 threads, and the corner outputs go to compact slots (the pairs of
+// each owned node, in corner order, after those of the nodes before it) instead
+// of 8 full corner buffers, so that four workgroups fit one CU.  The sums are
+// those of the dense form without its zero terms: the results are identical.
+// PART (OUT_CHEB): per-brick sums of b . x are written (A.partial set).
+template <typename T, int IN, int OUT, bool IO32 = false, bool SPARSE = false, bool PART = false>
+__global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
+                                                                      T* __restrict__ yout) {
     constexpr int RC = MvCfg<T>::RC, SX = 3 * RC, NCH = R / RC;
     using V = typename MvCfg<T>::V;
-    constexpr int NLD = (NH * 3 + MV_NT - 1) / MV_NT;
-    constexpr int NW = MV_NT / 64;
+    constexpr int NT = SPARSE ? MV_NT_SP : MV_NT;
+    constexpr int NLD = (NH * 3 + NT - 1) / NT;
+    constexpr int NW = NT / 64;
+    constexpr bool SUMS = OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && PART);
+    constexpr int NSLOT = SPARSE ? MV_SLOTS * 3 : (OB_BIAS + 8 * NO) * 3 + 64 + 2;
     __shared__ int s_node[NH];
     __shared__ int s_info[NET];
     __shared__ T s_E[NET];
     __shared__ unsigned char s_nm[NO];
-    __shared__ double s_wred[NCH][NW * 4][RC];   // per chunk, wave row and load case
+    __shared__ double s_wred[SUMS ? NCH : 1][NW * 4][RC];   // per chunk, wave row and load case
     __shared__ V s_x[NH * 3];
     // corner buffers (the bias keeps element bases non-negative), then a trash
     // area for the outputs of corners outside the owned nodes (lane l writes
     // vectors l .. l + 2; the contents are never read)
-    __shared__ V s_c[(OB_BIAS + 8 * NO) * 3 + 64 + 2];
+    __shared__ V s_c[NSLOT];
     __shared__ short s_hn[NH];                   // halo positions that hold a node, compact
     __shared__ unsigned char s_ol[NO];           // owned position of the brick's k-th solver node
-    __shared__ int s_wcnt[NW];
-    static_assert(NH <= MV_NT, "one thread per halo position");
+    __shared__ int s_wcnt[((NH + NT - 1) / NT) * NW];
+    __shared__ short s_sb[SPARSE ? NO : 1];      // SPARSE: first slot of each owned position
     static_assert(!IO32 || (sizeof(T) == 8 && IN == IN_V), "FP32 vectors with FP64 arithmetic");
 
-    const BrickInfo B = A.bricks[blockIdx.x];
+    const int bid = A.blist ? A.blist[A.boff + blockIdx.x] : (int)blockIdx.x;
+    const BrickInfo B = A.bricks[bid];
     if (A.done && A.done[B.dom]) return;
     const int tid = threadIdx.x;
-    const int* __restrict__ halo = A.halo + (int64_t)blockIdx.x * NH;
+    const int* __restrict__ halo = A.halo + (int64_t)bid * NH;
     const V* xv = reinterpret_cast<const V*>(xin);
     V* yv = reinterpret_cast<V*>(yout);
     const int c0 = (int)blockIdx.y * NCH / (int)gridDim.y, c1 = ((int)blockIdx.y + 1) * NCH / (int)gridDim.y;
@@ -320,7 +334,7 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
     if constexpr (EARLY_X) {
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
-            const int i = tid + k * MV_NT;
+            const int i = tid + k * NT;
             int o = -1;
             if (IN == IN_V && i < NH * 3) {
                 const int q = halo[i / 3];
@@ -332,44 +346,76 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
         if (IN == IN_V) prefetch(c0);
     }
     {
-        const int q = tid < NH ? halo[tid] : -1;
-        if (tid < NH) s_node[tid] = q;
-        const bool ex = q >= 0;
-        const int hx = tid % HX, hy = (tid / HX) % HY, hz = tid / (HX * HY);
-        if (ex && hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ)
-            s_ol[(q & NODE_MASK) - B.node_begin] = (unsigned char)((hx - 1) + BX * ((hy - 1) + BY * (hz - 1)));
-        if (tid < NO) s_nm[tid] = A.nmask[(int64_t)blockIdx.x * NO + tid];
+        // halo positions h = tid + k NT (one pass with 512 threads, two with 256)
+        constexpr int NHP = (NH + NT - 1) / NT;
+        bool exk[NHP];
+#pragma unroll
+        for (int kk = 0; kk < NHP; ++kk) {
+            const int h = tid + kk * NT;
+            const int q = h < NH ? halo[h] : -1;
+            if (h < NH) s_node[h] = q;
+            const bool ex = q >= 0;
+            exk[kk] = ex;
+            const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+            if (ex && hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ)
+                s_ol[(q & NODE_MASK) - B.node_begin] = (unsigned char)((hx - 1) + BX * ((hy - 1) + BY * (hz - 1)));
+        }
+        if (tid < NO) s_nm[tid] = A.nmask[(int64_t)bid * NO + tid];
         if constexpr (!EARLY_X) {
-            const unsigned long long bal = __ballot(ex);
+            // compact list of the halo positions that hold a node, in position order
             const int lane = tid & 63, wv = tid >> 6;
-            if (lane == 0) s_wcnt[wv] = __popcll(bal);
+            unsigned long long bal[NHP];
+#pragma unroll
+            for (int kk = 0; kk < NHP; ++kk) {
+                bal[kk] = __ballot(exk[kk]);
+                if (lane == 0) s_wcnt[kk * NW + wv] = __popcll(bal[kk]);
+            }
             __syncthreads();
-            int base = 0;
             nh = 0;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const int c = s_wcnt[w];
-                if (w < wv) base += c;
-                nh += c;
+            for (int kk = 0; kk < NHP; ++kk) {
+                int base = nh;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    const int c = s_wcnt[kk * NW + w];
+                    if (w < wv) base += c;
+                    nh += c;
+                }
+                if (exk[kk]) s_hn[base + __popcll(bal[kk] & ((1ull << lane) - 1))] = (short)(tid + kk * NT);
             }
-            if (ex) s_hn[base + __popcll(bal & ((1ull << lane) - 1))] = (short)tid;
         }
     }
     const int ne = B.e_cnt;
     {
         const T* __restrict__ eE = static_cast<const T*>(A.eE) + B.e_off;
         const int* __restrict__ ei = A.einfo + B.e_off;
-        for (int i = tid; i < ne; i += MV_NT) {
+        for (int i = tid; i < ne; i += NT) {
             s_info[i] = ei[i];
             s_E[i] = eE[i];
         }
     }
     __syncthreads();
+    if constexpr (SPARSE) {
+        // slot bases: exclusive prefix of the element counts of the owned positions
+        // (wave 0, two positions per lane); visible after the barrier below
+        if (tid < 64) {
+            const int c0 = __popc((unsigned)s_nm[2 * tid]), c1 = __popc((unsigned)s_nm[2 * tid + 1]);
+            int v = c0 + c1;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(v, o);
+                if (tid >= o) v += u;
+            }
+            const int ex = v - c0 - c1;
+            s_sb[2 * tid] = (short)ex;
+            s_sb[2 * tid + 1] = (short)(ex + c0);
+        }
+    }
 
     if constexpr (!EARLY_X) {
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
-            const int i = tid + k * MV_NT;
+            const int i = tid + k * NT;
             int o = -1, xi = 0;
             if (i < 3 * nh) {
                 const int h = s_hn[i / 3], d = i % 3;
@@ -391,7 +437,7 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
         } else {
             const DomInfo D = A.doms[B.dom];
             T* sx = reinterpret_cast<T*>(s_x);
-            for (int i = tid; i < nh * SX; i += MV_NT) {
+            for (int i = tid; i < nh * SX; i += NT) {
                 const int h = EARLY_X ? i / SX : (int)s_hn[i / SX], rem = i % SX, d = rem / RC, r = rem % RC;
                 const int lc = chunk * RC + r;
                 const int q = s_node[h];
@@ -413,14 +459,14 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
     };
 
     // chunk-invariant gather items of this thread: (owned solver node, direction)
-    // i = tid + j * MV_NT < 3 n_own
-    constexpr int NG = (NO * 3 + MV_NT - 1) / MV_NT;
+    // i = tid + j * NT < 3 n_own
+    constexpr int NG = (NO * 3 + NT - 1) / NT;
     int g_off[NG], g_xi[NG];
     unsigned g_m[NG];
     T e_di[NG];   // OUT_CHEB: Jacobi scaling of the item
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
-        const int i = tid + j * MV_NT;
+        const int i = tid + j * NT;
         int off = -1, xi = 0;
         unsigned m = 0;
         if (i < 3 * n_own) {
@@ -439,10 +485,11 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
 
     // corner slots that no element writes (element absent) are zero for the whole
     // launch, so the gather adds all 8 corner buffers without tests
-    for (int i = tid; i < n_own * 24; i += MV_NT) {
-        const int o = s_ol[i / 24], b = (i / 3) % 8, d = i % 3;
-        if (!((s_nm[o] >> b) & 1)) vzero(s_c[(OB_BIAS + b * NO + o) * 3 + d]);
-    }
+    if constexpr (!SPARSE)
+        for (int i = tid; i < n_own * 24; i += NT) {
+            const int o = s_ol[i / 24], b = (i / 3) % 8, d = i % 3;
+            if (!((s_nm[o] >> b) & 1)) vzero(s_c[(OB_BIAS + b * NO + o) * 3 + d]);
+        }
     stage(c0);
     if (IN == IN_V && c0 + 1 < c1) prefetch(c0 + 1);
     __syncthreads();
@@ -475,7 +522,7 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
             T* sc = reinterpret_cast<T*>(s_c);
             const int ntask = ne * RC;
 #pragma unroll 1
-            for (int t = tid; t < ntask; t += MV_NT) {
+            for (int t = tid; t < ntask; t += NT) {
                 const int r = t % RC, k = t / RC;
                 const int info = s_info[k];
                 const T E = s_E[k];
@@ -509,13 +556,27 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
                     }
                     fht_inv_tail(o[d]);
                 }
-                // corners outside the owned nodes write to this lane's trash slot
-                T* trash = sc + ((OB_BIAS + 8 * NO) * 3 + (tid & 63)) * RC;
+                if constexpr (SPARSE) {
+                    // owned corners only: slot of node ob = first slot + rank of corner b among its elements
+                    const int ob0 = ((info >> 9) & 255) - OB_BIAS;
 #pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    T* dst = (mask & (1u << b)) ? cb + (b * NO + ocorner(b)) * SX : trash;
+                    for (int b = 0; b < 8; ++b) {
+                        if (!(mask & (1u << b))) continue;
+                        const int ob = ob0 + ocorner(b);
+                        const int slot = s_sb[ob] + __popc((unsigned)s_nm[ob] & ((1u << b) - 1u));
+                        T* dst = sc + slot * SX + r;
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) dst[d * RC] = o[d][b];
+                        for (int d = 0; d < 3; ++d) dst[d * RC] = o[d][b];
+                    }
+                } else {
+                    // corners outside the owned nodes write to this lane's trash slot
+                    T* trash = sc + ((OB_BIAS + 8 * NO) * 3 + (tid & 63)) * RC;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        T* dst = (mask & (1u << b)) ? cb + (b * NO + ocorner(b)) * SX : trash;
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) dst[d * RC] = o[d][b];
+                    }
                 }
             }
         }
@@ -537,8 +598,15 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
             const int o = od & 127, d = od >> 7;
             V y;
             vzero(y);
+            if constexpr (SPARSE) {
+                const int sb = s_sb[o], cnt = __popc((unsigned)s_nm[o]);
 #pragma unroll
-            for (int b = 0; b < 8; ++b) vadd(y, s_c[(OB_BIAS + b * NO + o) * 3 + d]);
+                for (int b = 0; b < 8; ++b)
+                    if (b < cnt) vadd(y, s_c[(sb + b) * 3 + d]);
+            } else {
+#pragma unroll
+                for (int b = 0; b < 8; ++b) vadd(y, s_c[(OB_BIAS + b * NO + o) * 3 + d]);
+            }
             const bool bc = (m & 256u) != 0;
             const int vi = g_off[j] + chunk;
             if (OUT == OUT_MASKED_DOT) {
@@ -580,7 +648,7 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
                     }
                     if (A.cflags & CH_WRITE_D) *dp = dn;
                     y = xn;
-                    if (A.partial)
+                    if (PART)
 #pragma unroll
                         for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(bv, w) * (double)vcomp(y, w);
                 }
@@ -588,7 +656,7 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
             if constexpr (IO32) reinterpret_cast<float2*>(yout)[vi] = make_float2((float)y.x, (float)y.y);
             else yv[vi] = y;
         }
-        if (OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) {
+        if constexpr (SUMS) {
             // every lane holds the RC load cases of the chunk: fixed-order row sums
             // (waves past the last gather item hold zeros)
             const int lane = tid & 63, row = (tid >> 6) * 4 + (lane >> 4);
@@ -600,13 +668,13 @@ __global__ __launch_bounds__(MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T
         }
         __syncthreads();
     }
-    if ((OUT == OUT_MASKED_DOT || OUT == OUT_RESID || (OUT == OUT_CHEB && A.partial)) && tid < R &&
+    if (SUMS && tid < R &&
         tid / RC >= c0 && tid / RC < c1) {
         const int ch = tid / RC, r = tid % RC;
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < NW * 4; ++w) s += s_wred[ch][w][r];
-        A.partial[(int64_t)blockIdx.x * R + tid] = s;
+        A.partial[(int64_t)bid * R + tid] = s;
     }
 }
 
@@ -2190,6 +2258,45 @@ double sym_lmax(const double* A, int n) {
 
 // matvec tables of a brick list: tile element counts -> offsets -> halo table,
 // owned-node element masks and packed element lists (bricks re-uploaded with offsets)
+// (owned node, element corner) pairs of each brick: the population count of its
+// owned-node element masks
+__global__ __launch_bounds__(256) void k_brick_slots(const unsigned char* __restrict__ nmask, int nbr,
+                                                    int* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbr) return;
+    const unsigned* w = reinterpret_cast<const unsigned*>(nmask + (int64_t)b * NO);
+    int c = 0;
+#pragma unroll 8
+    for (int k = 0; k < NO / 4; ++k) c += __popc(w[k]);
+    out[b] = c;
+}
+
+// bricks of a level for the FP32 matvecs: those with at most MV_SLOTS corner
+// slots first (the sparse kernel), then the rest (the dense kernel)
+int brick_lists(hipStream_t s, int nbr, const DevMem& nmask, DevMem& blist, int* n_sparse) {
+    *n_sparse = 0;
+    if (nbr == 0) return DTC_OK;
+    DevMem cnt;
+    DTC_TRY(cnt.alloc((size_t)nbr * 4));
+    k_brick_slots<<<(nbr + 255) / 256, 256, 0, s>>>(nmask.as<unsigned char>(), nbr, cnt.as<int>());
+    DTC_HIP(hipGetLastError());
+    std::vector<int> hc(nbr), order;
+    DTC_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)nbr * 4, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipStreamSynchronize(s));
+    const char* se = std::getenv("DTC_SPARSE");
+    const bool sparse = !(se && se[0] == '0');
+    order.reserve(nbr);
+    for (int i = 0; i < nbr; ++i)
+        if (sparse && hc[i] <= MV_SLOTS) order.push_back(i);
+    *n_sparse = (int)order.size();
+    for (int i = 0; i < nbr; ++i)
+        if (!(sparse && hc[i] <= MV_SLOTS)) order.push_back(i);
+    DTC_TRY(blist.ensure((size_t)nbr * 4));
+    DTC_HIP(hipMemcpyAsync(blist.p, order.data(), (size_t)nbr * 4, hipMemcpyHostToDevice, s));
+    DTC_HIP(hipStreamSynchronize(s));
+    return DTC_OK;
+}
+
 int build_tables(hipStream_t s, std::vector<BrickInfo>& hb, DevMem& brick, const DevMem& dom, const DevMem& lat2node,
                  const DevMem& emod, DevMem& halo, DevMem& nmask, DevMem& einfo, DevMem& eE64, DevMem& eE32,
                  int64_t* n_tile) {
@@ -2320,6 +2427,7 @@ int batch_finalize(SolverBatch& b, int precision) {
         DTC_HIP(hipGetLastError());
         DTC_TRY(build_tables(s, b.h_brick, b.brick, b.dom, b.lat2node, b.emod, b.halo, b.nmask, b.einfo, b.eE64,
                              b.eE32, &b.n_tile_elems));
+        DTC_TRY(brick_lists(s, b.n_bricks, b.nmask, b.blist, &b.n_sparse));
     }
     DTC_HIP(hipStreamSynchronize(s));
     b.precision = precision;
@@ -2427,6 +2535,7 @@ int batch_build_mg(SolverBatch& b, int max_levels) {
             DTC_HIP(hipGetLastError());
             DTC_TRY(build_tables(s, C.h_brick, C.brick, C.dom, C.lat2node, C.emod, C.halo, C.nmask, C.einfo, C.eE64,
                                  C.eE32, &C.n_tile_elems));
+            DTC_TRY(brick_lists(s, C.n_bricks, C.nmask, C.blist, &C.n_sparse));
         }
         DTC_HIP(hipStreamSynchronize(s));   // ex, cnt go out of scope
     }
@@ -2497,8 +2606,8 @@ int harvest_events(SolverBatch& b) {
 template <typename T, int IN, int OUT>
 int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* part, const int* done, int split = 0) {
     if (b.n_bricks == 0) return DTC_OK;
-    MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(), b.einfo.as<int>(),
-             sizeof(T) == 4 ? b.eE32.p : b.eE64.p, done, part};
+    MvArgs a{b.brick.as<BrickInfo>(), nullptr, 0, b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(),
+             b.einfo.as<int>(), sizeof(T) == 4 ? b.eE32.p : b.eE64.p, done, part};
     // timed: the CG matvec (OUT_MASKED: of the block PCG) and the FP64 true residual
     constexpr int kind = (IN == IN_V && (OUT == OUT_MASKED_DOT || OUT == OUT_MASKED)) ? DTC_KSTAT_MV
                        : (IN == IN_U && OUT == OUT_RESID)                              ? DTC_KSTAT_RESID
@@ -2514,8 +2623,20 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
                                                              reinterpret_cast<const float*>(xin),
                                                              reinterpret_cast<float*>(yout));
         } else {
-            const dim3 grid(b.n_bricks, split > 0 ? split : tuned_split(TK_MV, 0, b.n_bricks, R / MvCfg<T>::RC));
-            k_matvec<T, IN, OUT><<<grid, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
+            const int gy = split > 0 ? split : tuned_split(TK_MV, 0, b.n_bricks, R / MvCfg<T>::RC);
+            if (b.blist.p) {
+                // sparse bricks (256-thread kernel), then the dense ones
+                a.blist = b.blist.as<int>();
+                a.boff = 0;
+                if (b.n_sparse > 0)
+                    k_matvec<T, IN, OUT, false, true><<<dim3(b.n_sparse, gy), MV_NT_SP, 0, b.stream>>>(a, wk, xin, yout);
+                a.boff = b.n_sparse;
+                if (b.n_bricks > b.n_sparse)
+                    k_matvec<T, IN, OUT, false, false><<<dim3(b.n_bricks - b.n_sparse, gy), MV_NT, 0, b.stream>>>(
+                        a, wk, xin, yout);
+            } else {
+                k_matvec<T, IN, OUT><<<dim3(b.n_bricks, gy), MV_NT, 0, b.stream>>>(a, wk, xin, yout);
+            }
         }
         DTC_HIP(hipGetLastError());
     } else {
@@ -2532,6 +2653,8 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
 struct MvTopo {
     int n_bricks;
     const BrickInfo* bricks;
+    const int* blist;              // sparse bricks first (n_sparse of them), then the dense ones; null: all dense
+    int n_sparse;
     const DomInfo* doms;
     const int* halo;
     const unsigned char* nmask;
@@ -2554,7 +2677,7 @@ int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float
                 float c2 = 0.f, double* partial = nullptr, int cflags = CH_READ_D | CH_WRITE_D,
                 SolverBatch* tb = nullptr, MfUse mf = MfUse{0, nullptr, 0.f}, int split = 0) {
     if (T.n_bricks == 0) return DTC_OK;
-    MvArgs a{T.bricks, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2, cflags};
+    MvArgs a{T.bricks, nullptr, 0, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2, cflags};
     const int tk = OUT == OUT_CHEB ? TK_CHEB : TK_BRES;
     const dim3 grid(T.n_bricks, split > 0 ? split : tuned_split(tk, mf.terms, T.n_bricks, R / MvCfg<float>::RC));
     constexpr int kind = OUT == OUT_CHEB ? DTC_KSTAT_CHEB : OUT == OUT_BRES ? DTC_KSTAT_BRES : -1;
@@ -2564,7 +2687,28 @@ int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float
     else if (mf.terms == 3) k_matvec_mf<OUT, 3><<<grid, MV_NT, 0, s>>>(a, mf.tab, mf.kmax, x, y);
     else if (mf.terms == 1 && partial) k_matvec_mf<OUT, 1, true><<<grid, MV_NT, 0, s>>>(a, mf.tab, mf.kmax, x, y);
     else if (mf.terms == 1) k_matvec_mf<OUT, 1><<<grid, MV_NT, 0, s>>>(a, mf.tab, mf.kmax, x, y);
-    else k_matvec<float, IN_V, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
+    else if (T.blist) {
+        // sparse bricks (256-thread kernel), then the dense ones
+        constexpr bool CP = OUT == OUT_CHEB;
+        MvArgs as = a;
+        as.blist = T.blist;
+        as.boff = 0;
+        if (T.n_sparse > 0) {
+            const dim3 gs(T.n_sparse, grid.y);
+            if (CP && partial) k_matvec<float, IN_V, OUT, false, true, CP><<<gs, MV_NT_SP, 0, s>>>(as, wk, x, y);
+            else k_matvec<float, IN_V, OUT, false, true, false><<<gs, MV_NT_SP, 0, s>>>(as, wk, x, y);
+        }
+        as.boff = T.n_sparse;
+        if (T.n_bricks > T.n_sparse) {
+            const dim3 gd(T.n_bricks - T.n_sparse, grid.y);
+            if (CP && partial) k_matvec<float, IN_V, OUT, false, false, CP><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
+            else k_matvec<float, IN_V, OUT, false, false, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
+        }
+    } else if (OUT == OUT_CHEB && partial) {
+        k_matvec<float, IN_V, OUT, false, false, OUT == OUT_CHEB><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
+    } else {
+        k_matvec<float, IN_V, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
+    }
     DTC_HIP(hipGetLastError());
     if (tb) {
         // x, y, b; Chebyshev: the direction read and written as the flags say
@@ -2621,7 +2765,7 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
     auto view = [&](int l) {
         LvView v;
         if (l == 0) {
-            v.T = {b.n_bricks, b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(),
+            v.T = {b.n_bricks, b.brick.as<BrickInfo>(), b.blist.as<int>(), b.n_sparse, b.dom.as<DomInfo>(), b.halo.as<int>(),
                    b.nmask.as<unsigned char>(), b.einfo.as<int>(), b.eE32.p};
             v.l2n = b.lat2node.as<int>();
             v.dinv = b.dinv32.as<float>();
@@ -2631,7 +2775,7 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
             v.d = b.d32.as<float>();
         } else {
             Level& C = b.mg[l - 1];
-            v.T = {C.n_bricks, C.brick.as<BrickInfo>(), C.dom.as<DomInfo>(), C.halo.as<int>(),
+            v.T = {C.n_bricks, C.brick.as<BrickInfo>(), C.blist.as<int>(), C.n_sparse, C.dom.as<DomInfo>(), C.halo.as<int>(),
                    C.nmask.as<unsigned char>(), C.einfo.as<int>(), C.eE32.p};
             v.l2n = C.lat2node.as<int>();
             v.dinv = C.dinv32.as<float>();
@@ -2835,7 +2979,7 @@ int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* di
 // for smooth directions FP32 sums lose most digits of A P to cancellation, and
 // W's small eigenvalues (the ones that steer the block) drown in that error.
 int launch_mv_q64(SolverBatch& b, const float* p, float* q, const int* done) {
-    MvArgs a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(),
+    MvArgs a{b.brick.as<BrickInfo>(), nullptr, 0, b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(),
              b.einfo.as<int>(), b.eE64.p, done, nullptr};
     int slot = -1;
     DTC_TRY(timed_begin(b, DTC_KSTAT_MV, &slot));
@@ -3175,13 +3319,15 @@ int autotune_splits(SolverBatch& b, const WK<float>& wk) {
         float *x, *bb, *y, *d;
         const float* dinv;
         if (l == 0) {
-            T = {b.n_bricks, b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(),
+            T = {b.n_bricks, b.brick.as<BrickInfo>(), b.blist.as<int>(), b.n_sparse, b.dom.as<DomInfo>(), b.halo.as<int>(),
+                 b.nmask.as<unsigned char>(),
                  b.einfo.as<int>(), b.eE32.p};
             x = b.z32.as<float>(); bb = b.r32.as<float>(); y = b.t32.as<float>(); d = b.d32.as<float>();
             dinv = b.dinv32.as<float>();
         } else {
             Level& C = b.mg[l - 1];
-            T = {C.n_bricks, C.brick.as<BrickInfo>(), C.dom.as<DomInfo>(), C.halo.as<int>(), C.nmask.as<unsigned char>(),
+            T = {C.n_bricks, C.brick.as<BrickInfo>(), C.blist.as<int>(), C.n_sparse, C.dom.as<DomInfo>(), C.halo.as<int>(),
+                 C.nmask.as<unsigned char>(),
                  C.einfo.as<int>(), C.eE32.p};
             x = C.x.as<float>(); bb = C.b.as<float>(); y = C.t.as<float>(); d = C.d.as<float>();
             dinv = C.dinv32.as<float>();

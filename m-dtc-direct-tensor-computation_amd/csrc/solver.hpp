@@ -18,6 +18,8 @@ struct Level {
     std::vector<DomInfo> h_dom;
     std::vector<BrickInfo> h_brick;
     DevMem dom, brick, lat2node, emod, halo, nmask, einfo, eE64, eE32, dinv64, dinv32;
+    DevMem blist;                  // bricks for the FP32 matvecs: n_sparse sparse ones first, then the dense ones
+    int n_sparse = 0;
     DevMem b, x, t, d;             // right-hand side, iterate, matvec output, Chebyshev direction
 };
 
@@ -90,6 +92,8 @@ struct SolverBatch {
     DevMem einfo;                  // int per tile element: halo base | (owned base + 41) << 9 | owned corners << 24
     DevMem eE64, eE32;             // element modulus per tile element
     int64_t n_tile_elems = 0;
+    DevMem blist;                  // bricks for the FP32 matvecs: n_sparse sparse ones first, then the dense ones
+    int n_sparse = 0;
     DevMem dinv64, dinv32;         // Jacobi: 1 / diag(K) per DOF [node][3] (0 on Dirichlet DOFs)
     // vectors, layout [node][3][24]
     DevMem x64, r64, p64, q64;

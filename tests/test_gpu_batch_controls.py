@@ -173,3 +173,24 @@ def test_mfma_element_products_match_the_walsh_kernel(monkeypatch, terms):
         assert _rel(c["C"], a["C"]) < 1e-7 and _rel(c["K_macro"], a["K_macro"]) < 1e-7
         if terms == "3":
             assert c["iterations"] <= 1.25 * a["iterations"] + 4, (a["iterations"], c["iterations"])
+
+
+def test_sparse_brick_kernel_equals_dense(monkeypatch):
+    """Bricks with at most 384 (owned node, element corner) pairs run the 256-thread
+    matvec with compact corner slots (the default); its sums are the dense kernel's
+    without the zero terms, so DTC_SPARSE=0 (every brick dense) gives the same
+    results bit for bit: binary and greyscale domains, multigrid block PCG and
+    Jacobi PCG."""
+    dtc = _dtc()
+    doms = [_binary(dtc, 24, 95), _binary(dtc, 28, 96), _grey(dtc, 20, 97)]
+    for pc in (dtc.DTC_PC_MG, dtc.DTC_PC_JACOBI):
+        kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=pc)
+        monkeypatch.setenv("DTC_SPARSE", "0")
+        rd = dtc.solve_batch(doms, **kw)
+        monkeypatch.delenv("DTC_SPARSE")
+        rs = dtc.solve_batch(doms, **kw)
+        for a, c in zip(rd, rs):
+            assert a["status"] == c["status"] == 0
+            assert a["iterations"] == c["iterations"]
+            for k in ("C", "K_macro", "ff"):
+                assert np.array_equal(a[k], c[k]), (pc, k)
