@@ -200,6 +200,21 @@ int dtc_rotation_search(const double EE[36], int criterion, double step_deg, dou
                         double* alpha, double n[3], double aa[9], double EE_opt[36],
                         int64_t* jj, double* crit);
 
+/* Rotation search that allocates nothing (no hipMalloc/hipFree, no stream
+ * creation): no counterpart in the reference.  For callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid it).
+ * Same arguments, results and errors as dtc_rotation_search; it runs on the
+ * null stream with its buffers in a fixed per-GPU work space of 256 KB, held
+ * until the process ends, which bounds the initial sweep to
+ * dtc_rotation_noalloc_max_points() grid points (about 33 M: an initial step
+ * of 0.45 degrees or more; 1 degree needs 25 KB).  A larger sweep is
+ * rejected with DTC_ERR_BAD_ARGUMENT.  Calls that overlap on one GPU share the
+ * work space and race: serialise them per GPU. */
+int dtc_rotation_search_noalloc(const double EE[36], int criterion, double step_deg, double num_zero,
+                                double* alpha, double n[3], double aa[9], double EE_opt[36],
+                                int64_t* jj, double* crit);
+int64_t dtc_rotation_noalloc_max_points(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -892,4 +892,22 @@ int dtc_rotation_search(const double EE[36], int criterion, double step_deg, dou
     return DTC_OK;
 }
 
+int dtc_rotation_search_noalloc(const double EE[36], int criterion, double step_deg, double num_zero, double* alpha,
+                                double n[3], double aa[9], double EE_opt[36], int64_t* jj, double* crit) {
+    if (!EE || !alpha || !n || !aa || !EE_opt) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL argument");
+    if (criterion != 1 && criterion != 2) return set_error(DTC_ERR_BAD_ARGUMENT, "criterion %d", criterion);
+    RotResult rr[2];
+    DTC_TRY(rotation_search_noalloc(EE, step_deg, num_zero > 0.0 ? num_zero : 1e-9, rr));
+    const RotResult& q = rr[criterion - 1];
+    *alpha = q.alpha;
+    std::memcpy(n, q.n, sizeof(q.n));
+    std::memcpy(aa, q.aa, sizeof(q.aa));
+    std::memcpy(EE_opt, q.EE_opt, sizeof(q.EE_opt));
+    if (jj) *jj = q.jj;
+    if (crit) *crit = q.crit;
+    return DTC_OK;
+}
+
+int64_t dtc_rotation_noalloc_max_points(void) { return rotation_noalloc_max_points(); }
+
 }  // extern "C"

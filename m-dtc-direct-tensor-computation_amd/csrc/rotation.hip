@@ -13,6 +13,7 @@
 // the same in one loop); ~3M points per domain are spread over the GPU and
 // reduced per block, then per domain.
 #include <cmath>
+#include <mutex>
 
 #include "dtc_internal.hpp"
 #include "rotation.hpp"
@@ -257,42 +258,50 @@ void rotation_finish(const double EE[36], const int64_t ang[3], int64_t jj, doub
     for (int k = 0; k < 9; ++k) out.aa[k] = aa[k];
 }
 
-int rotation_search_batch(const double* EE_host, int n_dom, double step_deg, double num_zero, hipStream_t s,
-                          std::vector<RotResult>& out) {
-    out.assign((size_t)n_dom * 2, RotResult{});
-    if (n_dom == 0) return DTC_OK;
-    if (!(step_deg > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "rotation step %g", step_deg);
+// Device buffers of one search: n_dom 6x6 tensors, per-block sweep minima
+// (bpd per domain and criterion), per-job minima and the refinement jobs.
+struct RotBufs {
+    double* EE;
+    double* pv;
+    int64_t* pi;
+    double* best;
+    int64_t* besti;
+    RotJob* jobs;
+};
+
+// Grid sizes of the initial sweep for an initial step of step_deg degrees.
+void rotation_grid(double step_deg, int64_t* na, int64_t* ne, int* bpd) {
+    *na = (int64_t)std::floor(180.0 / step_deg + 1e-9) + 1;
+    *ne = (int64_t)std::floor(90.0 / step_deg + 1e-9) + 1;
+    const int64_t npts = *na * *na * *ne;
+    *bpd = (int)((npts + (int64_t)NT * PPT - 1) / ((int64_t)NT * PPT));
+}
+
+// The search on caller-provided device buffers and host state (hb, prev: 2 n_dom
+// doubles; hbi, jjv: 2 n_dom int64; ang: 6 n_dom int64; J: 2 n_dom jobs), so that
+// a caller without device or host allocations can run it.
+int rotation_search_core(const double* EE_host, int n_dom, double step_deg, double num_zero, hipStream_t s,
+                         const RotBufs& d, double* hb, int64_t* hbi, double* prev, int64_t* jjv, int64_t* ang,
+                         RotJob* J, RotResult* out) {
     const double pi = std::acos(-1.0), pi_div_180 = pi / 180.0;
-    const int64_t na = (int64_t)std::floor(180.0 / step_deg + 1e-9) + 1;
-    const int64_t ne = (int64_t)std::floor(90.0 / step_deg + 1e-9) + 1;
-    const int64_t npts = na * na * ne;
-    const int bpd = (int)((npts + (int64_t)NT * PPT - 1) / ((int64_t)NT * PPT));
-    DevMem dEE, pv, pi_, best, besti, jobs;
-    DTC_TRY(dEE.alloc((size_t)n_dom * 36 * 8));
-    DTC_TRY(pv.alloc((size_t)n_dom * bpd * 2 * 8));
-    DTC_TRY(pi_.alloc((size_t)n_dom * bpd * 2 * 8));
-    DTC_TRY(best.alloc((size_t)n_dom * 2 * 8));
-    DTC_TRY(besti.alloc((size_t)n_dom * 2 * 8));
-    DTC_TRY(jobs.alloc((size_t)n_dom * 2 * sizeof(RotJob)));
-    DTC_HIP(hipMemcpyAsync(dEE.p, EE_host, (size_t)n_dom * 36 * 8, hipMemcpyHostToDevice, s));
-    k_sweep<<<(unsigned)(n_dom * bpd), NT, 0, s>>>(dEE.as<double>(), na, ne, step_deg * pi_div_180, bpd, pv.as<double>(),
-                                                   pi_.as<int64_t>());
-    k_sweep_reduce<<<(unsigned)(n_dom * 2), NT, 0, s>>>(pv.as<double>(), pi_.as<int64_t>(), bpd, best.as<double>(),
-                                                        besti.as<int64_t>());
+    int64_t na, ne;
+    int bpd;
+    rotation_grid(step_deg, &na, &ne, &bpd);
+    DTC_HIP(hipMemcpyAsync(d.EE, EE_host, (size_t)n_dom * 36 * 8, hipMemcpyHostToDevice, s));
+    k_sweep<<<(unsigned)(n_dom * bpd), NT, 0, s>>>(d.EE, na, ne, step_deg * pi_div_180, bpd, d.pv, d.pi);
+    k_sweep_reduce<<<(unsigned)(n_dom * 2), NT, 0, s>>>(d.pv, d.pi, bpd, d.best, d.besti);
     DTC_HIP(hipGetLastError());
-    std::vector<double> hb(n_dom * 2);
-    std::vector<int64_t> hbi(n_dom * 2);
-    DTC_HIP(hipMemcpyAsync(hb.data(), best.p, hb.size() * 8, hipMemcpyDeviceToHost, s));
-    DTC_HIP(hipMemcpyAsync(hbi.data(), besti.p, hbi.size() * 8, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipMemcpyAsync(hb, d.best, (size_t)n_dom * 2 * 8, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipMemcpyAsync(hbi, d.besti, (size_t)n_dom * 2 * 8, hipMemcpyDeviceToHost, s));
     DTC_HIP(hipStreamSynchronize(s));
     // refinement state per job (dom, criterion)
-    std::vector<RotJob> J(n_dom * 2);
-    std::vector<int64_t> ang(n_dom * 2 * 3), jjv(n_dom * 2, 1);
-    std::vector<double> prev(n_dom * 2);
     for (int k = 0; k < n_dom * 2; ++k) {
-        int64_t idx = hbi[k];
+        const int64_t idx = hbi[k];
         ang[3 * k] = idx % na; ang[3 * k + 1] = (idx / na) % na; ang[3 * k + 2] = idx / (na * na);
         prev[k] = hb[k];
+        jjv[k] = 1;
+        out[k] = RotResult{};
+        J[k] = RotJob{};
         J[k].dom = k / 2; J[k].criterion = 1 + (k & 1); J[k].active = 1;
     }
     double div = pi_div_180 * step_deg;
@@ -302,19 +311,19 @@ int rotation_search_batch(const double* EE_host, int n_dom, double step_deg, dou
         for (int k = 0; k < n_dom * 2; ++k) {
             if (!J[k].active) continue;
             any = true;
-            for (int d = 0; d < 3; ++d) J[k].s0[d] = (ang[3 * k + d] - 1) * 10;
+            for (int c = 0; c < 3; ++c) J[k].s0[c] = (ang[3 * k + c] - 1) * 10;
             J[k].div = div;
         }
         if (!any) break;
-        DTC_HIP(hipMemcpyAsync(jobs.p, J.data(), J.size() * sizeof(RotJob), hipMemcpyHostToDevice, s));
-        k_refine<<<(unsigned)(n_dom * 2), NT, 0, s>>>(dEE.as<double>(), jobs.as<RotJob>(), best.as<double>(), besti.as<int64_t>());
+        DTC_HIP(hipMemcpyAsync(d.jobs, J, (size_t)n_dom * 2 * sizeof(RotJob), hipMemcpyHostToDevice, s));
+        k_refine<<<(unsigned)(n_dom * 2), NT, 0, s>>>(d.EE, d.jobs, d.best, d.besti);
         DTC_HIP(hipGetLastError());
-        DTC_HIP(hipMemcpyAsync(hb.data(), best.p, hb.size() * 8, hipMemcpyDeviceToHost, s));
-        DTC_HIP(hipMemcpyAsync(hbi.data(), besti.p, hbi.size() * 8, hipMemcpyDeviceToHost, s));
+        DTC_HIP(hipMemcpyAsync(hb, d.best, (size_t)n_dom * 2 * 8, hipMemcpyDeviceToHost, s));
+        DTC_HIP(hipMemcpyAsync(hbi, d.besti, (size_t)n_dom * 2 * 8, hipMemcpyDeviceToHost, s));
         DTC_HIP(hipStreamSynchronize(s));
         for (int k = 0; k < n_dom * 2; ++k) {
             if (!J[k].active) continue;
-            int64_t i = hbi[k];
+            const int64_t i = hbi[k];
             ang[3 * k] = J[k].s0[0] + i % 21;
             ang[3 * k + 1] = J[k].s0[1] + (i / 21) % 21;
             ang[3 * k + 2] = J[k].s0[2] + i / 441;
@@ -329,6 +338,107 @@ int rotation_search_batch(const double* EE_host, int n_dom, double step_deg, dou
         rotation_finish(EE_host + 36 * (k / 2), &ang[3 * k], jjv[k], step_deg, out[k]);
     }
     return DTC_OK;
+}
+
+int rotation_search_batch(const double* EE_host, int n_dom, double step_deg, double num_zero, hipStream_t s,
+                          std::vector<RotResult>& out) {
+    out.assign((size_t)n_dom * 2, RotResult{});
+    if (n_dom == 0) return DTC_OK;
+    if (!(step_deg > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "rotation step %g", step_deg);
+    int64_t na, ne;
+    int bpd;
+    rotation_grid(step_deg, &na, &ne, &bpd);
+    DevMem dEE, pv, pi_, best, besti, jobs;
+    DTC_TRY(dEE.alloc((size_t)n_dom * 36 * 8));
+    DTC_TRY(pv.alloc((size_t)n_dom * bpd * 2 * 8));
+    DTC_TRY(pi_.alloc((size_t)n_dom * bpd * 2 * 8));
+    DTC_TRY(best.alloc((size_t)n_dom * 2 * 8));
+    DTC_TRY(besti.alloc((size_t)n_dom * 2 * 8));
+    DTC_TRY(jobs.alloc((size_t)n_dom * 2 * sizeof(RotJob)));
+    const RotBufs d{dEE.as<double>(), pv.as<double>(), pi_.as<int64_t>(), best.as<double>(), besti.as<int64_t>(),
+                    jobs.as<RotJob>()};
+    std::vector<double> hb(n_dom * 2), prev(n_dom * 2);
+    std::vector<int64_t> hbi(n_dom * 2), jjv(n_dom * 2), ang(n_dom * 2 * 3);
+    std::vector<RotJob> J(n_dom * 2);
+    return rotation_search_core(EE_host, n_dom, step_deg, num_zero, s, d, hb.data(), hbi.data(), prev.data(),
+                                jjv.data(), ang.data(), J.data(), out.data());
+}
+
+// ---------------------------------------------------------------- no-allocation search
+// Work space of the search that allocates nothing: one per GPU, held until the
+// process ends.  Host code never uses the array's name as an address (in host
+// code it names a host-side stand-in); it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[ROT_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// byte offsets of the buffers in the work space (one domain, 256-byte aligned)
+constexpr size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+size_t rot_scratch_layout(int bpd, size_t off[6]) {
+    const size_t sz[6] = {36 * 8, (size_t)bpd * 2 * 8, (size_t)bpd * 2 * 8, 2 * 8, 2 * 8, 2 * sizeof(RotJob)};
+    size_t o = 0;
+    for (int k = 0; k < 6; ++k) {
+        off[k] = o;
+        o = align256(o + sz[k]);
+    }
+    return o;
+}
+}  // namespace
+
+// device address of this GPU's work space (looked up once per device, with the
+// device current)
+int rotation_scratch(void** ptr) {
+    int dev = 0;
+    DTC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return set_error(DTC_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(DTC_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return DTC_OK;
+}
+
+int64_t rotation_noalloc_max_points() {
+    // largest sweep whose per-block minima fit the work space
+    size_t off[6];
+    int bpd = 1;
+    while (rot_scratch_layout(bpd + 1, off) <= ROT_SCRATCH_BYTES) ++bpd;
+    return (int64_t)bpd * NT * PPT;
+}
+
+int rotation_search_noalloc(const double EE[36], double step_deg, double num_zero, RotResult out[2]) {
+    if (!(step_deg > 0.0)) return set_error(DTC_ERR_BAD_ARGUMENT, "rotation step %g", step_deg);
+    int64_t na, ne;
+    int bpd;
+    rotation_grid(step_deg, &na, &ne, &bpd);
+    size_t off[6];
+    const size_t need = rot_scratch_layout(bpd, off);
+    if (need > ROT_SCRATCH_BYTES)
+        return set_error(DTC_ERR_BAD_ARGUMENT,
+                         "rotation step %g deg: %lld grid points need %zu bytes of work space, the fixed "
+                         "work space holds %zu (at most %lld points)",
+                         step_deg, (long long)(na * na * ne), need, (size_t)ROT_SCRATCH_BYTES,
+                         (long long)rotation_noalloc_max_points());
+    void* base = nullptr;
+    DTC_TRY(rotation_scratch(&base));
+    unsigned char* b = static_cast<unsigned char*>(base);
+    const RotBufs d{reinterpret_cast<double*>(b + off[0]), reinterpret_cast<double*>(b + off[1]),
+                    reinterpret_cast<int64_t*>(b + off[2]), reinterpret_cast<double*>(b + off[3]),
+                    reinterpret_cast<int64_t*>(b + off[4]), reinterpret_cast<RotJob*>(b + off[5])};
+    double hb[2], prev[2];
+    int64_t hbi[2], jjv[2], ang[6];
+    RotJob J[2];
+    // the null stream: no stream object is created
+    return rotation_search_core(EE, 1, step_deg, num_zero, nullptr, d, hb, hbi, prev, jjv, ang, J, out);
 }
 
 }  // namespace dtc

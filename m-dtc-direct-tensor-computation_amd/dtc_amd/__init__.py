@@ -300,12 +300,14 @@ def solve_batch(domains: Sequence[Domain], **kw):
     return [_result(r, d.macro_order) for r, d in zip(res, domains)]
 
 
-def rotation_search(EE, criterion, step_deg=1.0, num_zero=1e-9):
+def rotation_search(EE, criterion, step_deg=1.0, num_zero=1e-9, noalloc=False):
+    """CR_1 (criterion 1) / CR_2 (criterion 2) search.  noalloc: the entry point
+    that allocates nothing (fixed per-GPU work space, bounded sweep size)."""
     EEc = np.ascontiguousarray(np.asarray(EE, np.float64).T)
     alpha, jj, crit = C.c_double(), C.c_int64(), C.c_double()
     n, aa, EEo = np.zeros(3), np.zeros(9), np.zeros(36)
-    check("dtc_rotation_search", load().dtc_rotation_search(_p(EEc), int(criterion), float(step_deg), float(num_zero),
-                                                            C.byref(alpha), _p(n), _p(aa), _p(EEo), C.byref(jj),
-                                                            C.byref(crit)))
+    fn = "dtc_rotation_search_noalloc" if noalloc else "dtc_rotation_search"
+    check(fn, getattr(load(), fn)(_p(EEc), int(criterion), float(step_deg), float(num_zero),
+                                  C.byref(alpha), _p(n), _p(aa), _p(EEo), C.byref(jj), C.byref(crit)))
     return dict(alpha=alpha.value, n=n, aa=aa.reshape(3, 3).T.copy(), EE_opt=EEo.reshape(6, 6).T.copy(),
                 jj=jj.value, crit=crit.value)

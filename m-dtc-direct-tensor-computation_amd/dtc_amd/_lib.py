@@ -83,6 +83,8 @@ SIGNATURES = {
     "dtc_batch_kernel_stats": (C.c_int, [P, C.c_int, P, P, P]),
     "dtc_batch_destroy": (C.c_int, [P]),
     "dtc_rotation_search": (C.c_int, [P, C.c_int, C.c_double, C.c_double, P, P, P, P, P, P]),
+    "dtc_rotation_search_noalloc": (C.c_int, [P, C.c_int, C.c_double, C.c_double, P, P, P, P, P, P]),
+    "dtc_rotation_noalloc_max_points": (I64, []),
 }
 
 _lib = None
