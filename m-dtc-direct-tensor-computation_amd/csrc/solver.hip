@@ -2283,8 +2283,10 @@ int brick_lists(hipStream_t s, int nbr, const DevMem& nmask, DevMem& blist, int*
     std::vector<int> hc(nbr), order;
     DTC_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)nbr * 4, hipMemcpyDeviceToHost, s));
     DTC_HIP(hipStreamSynchronize(s));
+    // DTC_SPARSE=1: bricks with few corner slots run the 256-thread variant (measured
+    // 10% slower over the whole mixed solve on trabecular batches, so not the default)
     const char* se = std::getenv("DTC_SPARSE");
-    const bool sparse = !(se && se[0] == '0');
+    const bool sparse = se && se[0] == '1';
     order.reserve(nbr);
     for (int i = 0; i < nbr; ++i)
         if (sparse && hc[i] <= MV_SLOTS) order.push_back(i);
@@ -3057,7 +3059,8 @@ int block_iteration(SolverBatch& b, const WK<float>& wk, float* x, float* r, flo
     const int D = b.n_dom, NBk = b.n_bricks;
     int* done = b.done.as<int>();
     const BrickInfo* bl = b.brick.as<BrickInfo>();
-    if (b.block_q64) DTC_TRY(launch_mv_q64(b, p, q, done));
+    if (b.block_q64 && b.micro20) DTC_TRY(mv20_q64(b, p, q, done));
+    else if (b.block_q64) DTC_TRY(launch_mv_q64(b, p, q, done));
     else DTC_TRY((launch_mv<float, IN_V, OUT_MASKED>(b, wk, p, q, nullptr, done)));
     DTC_TRY(block_gram(b, p, q, r));
     k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 0, b.lfac.as<double>(), b.bcoef.as<float>());
@@ -3454,9 +3457,13 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
         float* r32 = b.r32.as<float>();
         float* p32 = b.p32.as<float>();
         float* z32 = mg ? b.z32.as<float>() : nullptr;
-        // the block's A P in FP32 (DTC_BLOCK_Q64=1: FP64 arithmetic)
+        // the block's A P in FP64 arithmetic (FP32 vectors) for HEX20 micro meshes,
+        // in FP32 for HEX08 (DTC_BLOCK_Q64=1 / 0 overrides).  HEX20 rows sum up to
+        // 480 products of mixed sign; in FP32 that sum loses the small eigenvalues
+        // of W = P^T A P, and on meshes with corner- or edge-hinged voxel clusters
+        // (near-mechanisms) the block then drifts along them and diverges.
         const char* bq64 = std::getenv("DTC_BLOCK_Q64");
-        b.block_q64 = bq64 && bq64[0] == '1';
+        b.block_q64 = bq64 ? bq64[0] == '1' : b.micro20;
         const char* brs = std::getenv("DTC_BLOCK_RESTART");
         const bool block_restart = !(brs && brs[0] == '0');
         // a block phase ends at its targets; the cap only bounds FP32 drift

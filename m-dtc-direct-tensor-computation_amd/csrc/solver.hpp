@@ -136,6 +136,9 @@ struct SolverBatch {
     std::vector<Dom20> d20;        // per domain group (empty entries for failed groups)
     DevMem d20dev;                 // Dom20Dev[]
     std::vector<Dom20Dev> h_d20dev;   // host copy
+    DevMem ye20;                   // [element][60][24] element forces (FP32 or FP64) of the two-pass operator
+    DevMem ebase20;                // int64 [group]: first element of each domain group in ye20
+    std::vector<int64_t> h_ebase20;
     std::vector<double> khat20;    // E = 1 element stiffness, 60 x 60
     DevMem k20_64, k20_32;
     WalshK wk64{};
@@ -213,6 +216,8 @@ int batch_add_domain20(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& 
                        double escale, int lc_base, hipStream_t s);
 int finalize20(SolverBatch& b);
 int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout, double* part, const int* done);
+// Q = K P of the block PCG on a HEX20 batch: FP32 vectors, FP64 arithmetic (masked rows)
+int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done);
 int project20(SolverBatch& b);
 int fields20(SolverBatch& b, int i, int ncol, double* u_dev, double* r_dev);
 // sensitivity dL/dE per voxel of domain i.

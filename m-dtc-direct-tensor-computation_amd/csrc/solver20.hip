@@ -125,9 +125,10 @@ struct Mv20 {
 
 // y = K x on the pseudo-bricks; modes as the HEX08 kernel (IN_V / IN_U input,
 // MASKED_DOT / RESID / PLAIN / MASKED output)
-template <typename T, int IN, int OUT>
-__global__ __launch_bounds__(NT20) void k_matvec20(Mv20 A, const T* __restrict__ Kg, const T* __restrict__ xin,
-                                                   T* __restrict__ yout) {
+// VT: the vectors' type (T, or float with T = double: FP32 vectors, FP64 arithmetic)
+template <typename T, int IN, int OUT, typename VT = T>
+__global__ __launch_bounds__(NT20) void k_matvec20(Mv20 A, const T* __restrict__ Kg, const VT* __restrict__ xin,
+                                                   VT* __restrict__ yout) {
     __shared__ T sK[3600];
     __shared__ double s_red[NT20];
     const BrickInfo B = A.bricks[blockIdx.x];
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(NT20) void k_matvec20(Mv20 A, const T* __restrict__
                 for (int d2 = 0; d2 < 3; ++d2) {
                     if (IN == 1 && G.bcf[qb]) xv[d2] = (T)bc20(D, G, qb, d2, r);
                     else if (IN == 1 && !xin) xv[d2] = T(0);
-                    else xv[d2] = xin[((int64_t)(G.node_begin + qb) * 3 + d2) * R + r];
+                    else xv[d2] = (T)xin[((int64_t)(G.node_begin + qb) * 3 + d2) * R + r];
                 }
 #pragma unroll
                 for (int dd = 0; dd < 3; ++dd)
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(NT20) void k_matvec20(Mv20 A, const T* __restrict__
             } else if (OUT == 3) {
                 if (bc) v = T(0);
             }
-            yout[o] = v;
+            yout[o] = (VT)v;
         }
     }
     if (OUT == 0 || OUT == 1) {
@@ -258,6 +259,193 @@ __global__ __launch_bounds__(NT20) void k_fields20(const BrickInfo* __restrict__
     }
 }
 
+// ---------------------------------------------------------------- element-wise FP32 operator
+// y = K x in two passes (FP32, IN_V): k20_elem forms every element's forces
+// y_e = E_e Khat x_e for the 24 load cases into a per-element buffer (60 rows x
+// 24 load cases), k20_gather sums them per node over the node's sorted
+// (element, local node) list.  The products and sums run in the node-gather
+// kernel's order (k = 3 b + d per row, elements in list order).
+constexpr int E20_EPB = 4;                 // elements per workgroup
+constexpr int E20_NT = 240;                // 24 load cases x 10 row groups of 6 rows
+
+__global__ __launch_bounds__(E20_NT) void k20_elem(Dom20Dev G, int dom, const int* __restrict__ done,
+                                                  const float* __restrict__ Kg, const float* __restrict__ x,
+                                                  float* __restrict__ ye, int64_t ne) {
+    __shared__ float sK[3600];
+    __shared__ float sx[E20_EPB][60][R];
+    if (done && done[dom]) return;
+    const int tid = threadIdx.x;
+    const int64_t e0 = (int64_t)blockIdx.x * E20_EPB;
+    for (int i = tid; i < 3600; i += E20_NT) sK[i] = Kg[i];
+    // x of the block's elements: 20 nodes x 72 contiguous values each
+    for (int i = tid; i < E20_EPB * 20 * 72; i += E20_NT) {
+        const int el = i / (20 * 72), rem = i % (20 * 72), a = rem / 72, v = rem % 72;
+        const int64_t e = e0 + el;
+        float val = 0.f;
+        if (e < ne) {
+            const int q = G.elems[e * 20 + a];
+            if (q >= 0) val = x[(G.node_begin + q) * 72 + v];
+        }
+        sx[el][3 * a + v / R][v % R] = val;
+    }
+    __syncthreads();
+    const int r = tid % R, g = tid / R;
+    float acc[E20_EPB][6];
+#pragma unroll
+    for (int el = 0; el < E20_EPB; ++el)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc[el][j] = 0.f;
+    for (int k = 0; k < 60; ++k) {
+        float xk[E20_EPB], kk[6];
+#pragma unroll
+        for (int el = 0; el < E20_EPB; ++el) xk[el] = sx[el][k][r];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) kk[j] = sK[(g * 6 + j) * 60 + k];
+#pragma unroll
+        for (int el = 0; el < E20_EPB; ++el)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) acc[el][j] += kk[j] * xk[el];
+    }
+#pragma unroll
+    for (int el = 0; el < E20_EPB; ++el) {
+        const int64_t e = e0 + el;
+        if (e >= ne) break;
+        const float E = (float)G.emod[e];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) ye[(e * 60 + g * 6 + j) * R + r] = E * acc[el][j];
+    }
+}
+
+// The same element forces on the matrix cores, FP32 (v_mfma_f32_16x16x4_f32) or
+// FP64 (v_mfma_f64_16x16x4_f64) products and sums: for a group of EM elements,
+// Y (64 x 24 EM) = Khat (60 x 60, rows padded to 64) X (60 x 24 EM), column
+// c = 24 e + load case.  Wave w owns rows 16 w .. 16 w + 15 and keeps its Khat
+// fragments in registers for the whole block; X (FP32 vectors, widened to T) is
+// staged in LDS with a padded row stride.  Lane layout of the 16x16x4 products
+// (l = lane, measured on gfx950): A[i][k] in lane i + 16 k, B[k][j] in lane
+// j + 16 k; D[i][j] in lane j + 16 (i / 4), component i % 4 (f32), and in lane
+// j + 16 (i % 4), component i / 4 (f64).
+typedef float em_float4 __attribute__((ext_vector_type(4)));
+typedef double em_double4 __attribute__((ext_vector_type(4)));
+template <typename T> struct EmCfg;
+template <> struct EmCfg<float> {
+    static constexpr int EM = 8;                 // elements per workgroup: 192 columns, 12 tiles
+    static constexpr int XS = 24 * EM + 16;      // LDS row stride (the 32-lane halves of a read hit distinct banks)
+    using V4 = em_float4;
+    __device__ static V4 mfma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int drow(int k4, int c) { return 4 * k4 + c; }   // row of D component c in lane group k4
+};
+template <> struct EmCfg<double> {
+    static constexpr int EM = 4;                 // 96 columns, 6 tiles
+    static constexpr int XS = 24 * EM + 8;
+    using V4 = em_double4;
+    __device__ static V4 mfma(double a, double b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int drow(int k4, int c) { return k4 + 4 * c; }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k20_elem_mf(Dom20Dev G, int dom, const int* __restrict__ done,
+                                                   const T* __restrict__ Kg, const float* __restrict__ x,
+                                                   T* __restrict__ ye, int64_t ne) {
+    constexpr int EM = EmCfg<T>::EM, XS = EmCfg<T>::XS, NCT = 24 * EM / 16;
+    using V4 = typename EmCfg<T>::V4;
+    __shared__ T sX[60 * XS];
+    __shared__ T sE[EM];
+    if (done && done[dom]) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t e0 = (int64_t)blockIdx.x * EM;
+    // X: 20 nodes x 72 contiguous floats per element, read as float4 (18 per node)
+    for (int i = tid; i < EM * 20 * 18; i += 256) {
+        const int el = i / 360, rem = i % 360, a = rem / 18, v = 4 * (rem % 18), d = v / R, r = v % R;
+        const int64_t e = e0 + el;
+        float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (e < ne) {
+            const int q = G.elems[e * 20 + a];
+            if (q >= 0) val = *reinterpret_cast<const float4*>(x + (G.node_begin + q) * 72 + v);
+        }
+        T* dst = sX + (3 * a + d) * XS + el * R + r;
+        dst[0] = (T)val.x;
+        dst[1] = (T)val.y;
+        dst[2] = (T)val.z;
+        dst[3] = (T)val.w;
+    }
+    if (tid < EM) sE[tid] = e0 + tid < ne ? (T)G.emod[e0 + tid] : T(0);
+    // Khat fragments of this wave's row tile: A[i][k] = Khat[16 w + i][4 ks + k]
+    const int i16 = lane & 15, k4 = lane >> 4;
+    T a[15];
+    const int row = 16 * w + i16;
+#pragma unroll
+    for (int ks = 0; ks < 15; ++ks) a[ks] = row < 60 ? Kg[row * 60 + 4 * ks + k4] : T(0);
+    __syncthreads();
+    V4 acc[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) acc[ct] = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int ks = 0; ks < 15; ++ks) {
+        const T* xb = sX + (4 * ks + k4) * XS + i16;
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) acc[ct] = EmCfg<T>::mfma(a[ks], xb[16 * ct], acc[ct]);
+    }
+    // D: row 16 w + drow(lane / 16, c), column 16 ct + lane % 16
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+        const int col = 16 * ct + i16, el = col / R, r = col % R;
+        const int64_t e = e0 + el;
+        if (e >= ne) continue;
+        const T E = sE[el];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int rr = 16 * w + EmCfg<T>::drow(k4, c);
+            if (rr < 60) ye[(e * 60 + rr) * R + r] = E * acc[ct][c];
+        }
+    }
+}
+
+// y on the pseudo-bricks: the sum of the adjacent elements' forces; MASKED zeroes
+// Dirichlet rows, MASKED_DOT also writes per-brick sums of x . y
+// (T: the type of the element forces and of the sums; the vectors are FP32)
+template <typename T, int OUT>
+__global__ __launch_bounds__(NT20) void k20_gather(const BrickInfo* __restrict__ bricks,
+                                                  const Dom20Dev* __restrict__ d20,
+                                                  const int64_t* __restrict__ ebase, const int* __restrict__ done,
+                                                  const T* __restrict__ ye, const float* __restrict__ xin,
+                                                  float* __restrict__ yout, double* __restrict__ partial) {
+    __shared__ double s_red[NT20];
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done && done[B.dom]) return;
+    const Dom20Dev G = d20[B.dom];
+    const int64_t eb = ebase[B.dom];
+    const int tid = threadIdx.x, r = tid % R, g = tid / R;
+    double acc = 0.0;
+    for (int q = B.node_begin + g; q < B.node_end; q += NT20 / R) {
+        const int ql = (int)(q - G.node_begin);
+        T y[3] = {T(0), T(0), T(0)};
+        for (int k = G.adj_off[ql]; k < G.adj_off[ql + 1]; ++k) {
+            const int e = G.adj[k] >> 5, a = G.adj[k] & 31;
+            const T* src = ye + ((eb + e) * 60 + 3 * a) * R + r;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) y[d] += src[d * R];
+        }
+        const bool bc = G.bcf[ql] != 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int64_t o = ((int64_t)q * 3 + d) * R + r;
+            const float v = bc ? 0.f : (float)y[d];
+            if (OUT == 0) acc += (double)xin[o] * (double)v;
+            yout[o] = v;
+        }
+    }
+    if (OUT == 0) {
+        s_red[tid] = acc;
+        __syncthreads();
+        if (tid < R) {
+            double sum = 0.0;
+            for (int gg = 0; gg < NT20 / R; ++gg) sum += s_red[gg * R + tid];
+            partial[(int64_t)blockIdx.x * R + tid] = sum;
+        }
+    }
+}
+
 inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 template <typename T, int IN, int OUT>
@@ -277,11 +465,64 @@ int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout
         if (in == 0 && out == 0) return mv20_launch<double, 0, 0>(b, xin, yout, part, done);
         if (in == 1 && out == 1) return mv20_launch<double, 1, 1>(b, xin, yout, part, done);
         if (in == 1 && out == 2) return mv20_launch<double, 1, 2>(b, xin, yout, part, done);
-    } else {
-        if (in == 0 && out == 0) return mv20_launch<float, 0, 0>(b, xin, yout, part, done);
-        if (in == 0 && out == 3) return mv20_launch<float, 0, 3>(b, xin, yout, part, done);
+    } else if (in == 0 && (out == 0 || out == 3)) {
+        const char* ee = std::getenv("DTC_HEX20_GATHER");
+        if (ee && ee[0] == '1') {   // the node-gather kernel (reference form of the operator)
+            if (out == 0) return mv20_launch<float, 0, 0>(b, xin, yout, part, done);
+            return mv20_launch<float, 0, 3>(b, xin, yout, part, done);
+        }
+        if (b.n_bricks == 0) return DTC_OK;
+        // element products on the matrix cores (DTC_HEX20_MFMA=0: the VALU form)
+        const char* me = std::getenv("DTC_HEX20_MFMA");
+        const bool use_mf = !(me && me[0] == '0');
+        hipStream_t s = b.stream;
+        for (int g = 0; g < b.n_dom; ++g) {
+            const Dom20Dev& G = b.h_d20dev[g];
+            const int64_t ne = b.h_elems[g];   // the count ye20 was sized for (0 for a domain that failed set-up)
+            if (ne <= 0 || !G.elems) continue;
+            float* ye = b.ye20.as<float>() + b.h_ebase20[g] * 60 * R;
+            if (use_mf)
+                k20_elem_mf<float><<<nb(ne, EmCfg<float>::EM), 256, 0, s>>>(G, g, done, b.k20_32.as<float>(),
+                                                                            (const float*)xin, ye, ne);
+            else
+                k20_elem<<<nb(ne, E20_EPB), E20_NT, 0, s>>>(G, g, done, b.k20_32.as<float>(), (const float*)xin, ye, ne);
+        }
+        if (out == 0)
+            k20_gather<float, 0><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
+                                                      b.ebase20.as<int64_t>(), done, b.ye20.as<float>(),
+                                                      (const float*)xin, (float*)yout, part);
+        else
+            k20_gather<float, 3><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
+                                                      b.ebase20.as<int64_t>(), done, b.ye20.as<float>(),
+                                                      (const float*)xin, (float*)yout, part);
+        DTC_HIP(hipGetLastError());
+        return DTC_OK;
     }
     return set_error(DTC_ERR_UNSUPPORTED, "HEX20 micro operator: mode (%d, %d)", in, out);
+}
+
+int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done) {
+    if (b.n_bricks == 0) return DTC_OK;
+    hipStream_t s = b.stream;
+    const char* ee = std::getenv("DTC_HEX20_GATHER");
+    if (ee && ee[0] == '1') {   // the node-gather form
+        Mv20 a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.d20dev.as<Dom20Dev>(), done, nullptr};
+        k_matvec20<double, 0, 3, float><<<b.n_bricks, NT20, 0, s>>>(a, b.k20_64.as<double>(), p, q);
+        DTC_HIP(hipGetLastError());
+        return DTC_OK;
+    }
+    for (int g = 0; g < b.n_dom; ++g) {
+        const Dom20Dev& G = b.h_d20dev[g];
+        const int64_t ne = b.h_elems[g];
+        if (ne <= 0 || !G.elems) continue;
+        k20_elem_mf<double><<<nb(ne, EmCfg<double>::EM), 256, 0, s>>>(G, g, done, b.k20_64.as<double>(), p,
+                                                                      b.ye20.as<double>() + b.h_ebase20[g] * 60 * R, ne);
+    }
+    k20_gather<double, 3><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
+                                                      b.ebase20.as<int64_t>(), done, b.ye20.as<double>(), p, q,
+                                                      nullptr);
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
 }
 
 int batch_add_domain20(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& d, const double* modulus_dev,
@@ -391,6 +632,18 @@ int finalize20(SolverBatch& b) {
                         G.Lx, G.Ly, G.Lz};
     }
     b.h_d20dev = h;
+    // per-element force buffer of the FP32 operator: element offsets per domain group
+    b.h_ebase20.assign(b.n_dom, 0);
+    int64_t ne_all = 0;
+    for (int i = 0; i < b.n_dom; ++i) {
+        b.h_ebase20[i] = ne_all;
+        ne_all += b.h_elems[i];
+    }
+    DTC_TRY(b.ebase20.ensure(std::max<size_t>(b.h_ebase20.size(), 1) * 8));
+    if (!b.h_ebase20.empty())
+        DTC_HIP(hipMemcpyAsync(b.ebase20.p, b.h_ebase20.data(), b.h_ebase20.size() * 8, hipMemcpyHostToDevice, s));
+    // FP64 element forces of the block's Q = K P, FP32 ones of the smoothing steps (one buffer)
+    DTC_TRY(b.ye20.ensure((size_t)std::max<int64_t>(ne_all, 1) * 60 * R * 8));
     DTC_TRY(b.d20dev.ensure(std::max<size_t>(h.size(), 1) * sizeof(Dom20Dev)));
     if (!h.empty()) DTC_HIP(hipMemcpyAsync(b.d20dev.p, h.data(), h.size() * sizeof(Dom20Dev), hipMemcpyHostToDevice, s));
     std::vector<float> k32(3600);

@@ -176,21 +176,23 @@ def test_mfma_element_products_match_the_walsh_kernel(monkeypatch, terms):
 
 
 def test_sparse_brick_kernel_equals_dense(monkeypatch):
-    """Bricks with at most 384 (owned node, element corner) pairs run the 256-thread
-    matvec with compact corner slots (the default); its sums are the dense kernel's
-    without the zero terms, so DTC_SPARSE=0 (every brick dense) gives the same
-    results bit for bit: binary and greyscale domains, multigrid block PCG and
-    Jacobi PCG."""
+    """DTC_SPARSE=1 runs bricks with at most 384 (owned node, element corner) pairs
+    through the 256-thread matvec with compact corner slots; its corner sums skip
+    the zero terms of the dense kernel and the compiler contracts its products
+    differently, so results agree to FP32 rounding: 1e-7 on the tensors, iteration
+    counts within 5 % (the hundreds of FP32 Jacobi-PCG iterations drift apart by
+    a few): binary and greyscale domains, multigrid block PCG and Jacobi PCG."""
     dtc = _dtc()
     doms = [_binary(dtc, 24, 95), _binary(dtc, 28, 96), _grey(dtc, 20, 97)]
     for pc in (dtc.DTC_PC_MG, dtc.DTC_PC_JACOBI):
         kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=pc)
         monkeypatch.setenv("DTC_SPARSE", "0")
         rd = dtc.solve_batch(doms, **kw)
-        monkeypatch.delenv("DTC_SPARSE")
+        monkeypatch.setenv("DTC_SPARSE", "1")
         rs = dtc.solve_batch(doms, **kw)
         for a, c in zip(rd, rs):
             assert a["status"] == c["status"] == 0
-            assert a["iterations"] == c["iterations"]
+            assert abs(a["iterations"] - c["iterations"]) <= 2 + 0.05 * a["iterations"]
             for k in ("C", "K_macro", "ff"):
-                assert np.array_equal(a[k], c[k]), (pc, k)
+                ref = np.abs(a[k]).max()
+                assert np.abs(a[k] - c[k]).max() <= 1e-7 * ref, (pc, k)

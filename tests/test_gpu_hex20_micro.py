@@ -144,3 +144,50 @@ def test_hex20_pmultigrid_cuts_iterations_greyscale():
     assert rj["status"] == 0 == rm["status"] and rm["relres"] <= 1e-8
     assert _rel(rm["C"], rj["C"]) < 1e-6 and _rel(rm["K_macro"], rj["K_macro"]) < 1e-6
     assert rm["iterations"] < rj["iterations"] / 5, (rm["iterations"], rj["iterations"])
+
+
+@pytest.mark.parametrize("mfma", ["1", "0"])
+@pytest.mark.parametrize("precond", ["jacobi", "mg"])
+def test_hex20_element_operator_equals_node_gather(monkeypatch, precond, mfma):
+    """The FP32 HEX20 operator in two passes (element forces Khat x_e on the matrix
+    cores or on the VALU, then a sum per node) against the node-gather form of
+    the same product: they differ only in FP32 rounding (the order of the 60
+    products of a row, E * (Khat x_e) rounded before the sum over a node's
+    elements), so tensors agree to 1e-6 and the iteration counts within 5 %."""
+    monkeypatch.setenv("DTC_HEX20_MFMA", mfma)
+    dtc = _dtc()
+    from dtc_amd import synth
+    phi = synth.trabecular(12, seed=7, sigma=1.5, bvtv=0.4)
+    d = dtc.Domain(phi, delta=(0.01,) * 3, xa_n=(2, 2, 2), micro_nen=20)
+    pc = dtc.DTC_PC_MG if precond == "mg" else dtc.DTC_PC_JACOBI
+    out = {}
+    for gather in ("1", "0"):
+        monkeypatch.setenv("DTC_HEX20_GATHER", gather)
+        out[gather] = dtc.solve_domain(d, precision=1, rel_tol=1e-8, rotation=False, precond=pc)
+    a, b = out["1"], out["0"]
+    assert a["status"] == 0 and b["status"] == 0 and b["relres"] <= 1e-8
+    for key in ("ff", "K_macro", "C"):
+        assert _rel(b[key], a[key]) < 1e-6, key
+    assert abs(b["iterations"] - a["iterations"]) <= max(2, 0.05 * a["iterations"])
+
+
+def test_hex20_hinged_clusters_converge_in_the_mixed_solve():
+    """A 48^3 trabecular HEX20 domain whose solid holds voxel clusters attached to
+    the rest only through an edge or a corner (near-mechanisms): the block PCG
+    with the p-multigrid converges in the mixed precision (no FP64 fallback).
+    With the block's Q = K P in FP32 arithmetic (DTC_BLOCK_Q64=0) this domain
+    diverged after the first residual replacement."""
+    dtc = _dtc()
+    from dtc_amd import synth
+    from scipy import ndimage
+    phi = synth.trabecular(48, seed=4, sigma=2.0, bvtv=0.2)
+    lab6, n6 = ndimage.label(phi > 0)                          # face-connected parts
+    lab26, n26 = ndimage.label(phi > 0, structure=np.ones((3, 3, 3)))
+    assert n26 == 1 and n6 > 1                                  # hinged parts are present
+    d = dtc.Domain(phi, delta=(0.01,) * 3, micro_nen=20)
+    r = dtc.solve_domain(d, precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+    assert r["status"] == 0 and r["relres"] <= 1e-8
+    assert r["precision_used"] == dtc.DTC_FP32_REFINE
+    assert r["iterations"] <= 150, r["iterations"]
+    K = r["K_macro"]
+    assert np.abs(K - K.T).max() < 1e-6 * np.abs(K).max()
