@@ -259,14 +259,15 @@ constexpr int CH_READ_D = 1, CH_D_FROM_X = 2, CH_WRITE_D = 4;
 // and, when partial is set, writes per-brick sums of b . yout.
 // launch bounds: at least 4 waves per SIMD (2 workgroups of 8 waves per CU), which
 // caps the kernel at 128 VGPRs
-// IO32 (T = double, IN_V only): x and y are FP32 vectors, the arithmetic is FP64
+// IO32 (T = double, IN_V only): 1 = x and y are FP32 vectors, 2 = x is an FP32
+// vector and y an FP64 one; the arithmetic is FP64
 // SPARSE: the variant for bricks with at most MV_SLOTS (owned node, element corner)
 // pairs: 256 threads, and the corner outputs go to compact slots (the pairs of
 // each owned node, in corner order, after those of the nodes before it) instead
 // of 8 full corner buffers, so that four workgroups fit one CU.  The sums are
 // those of the dense form without its zero terms: the results are identical.
 // PART (OUT_CHEB): per-brick sums of b . x are written (A.partial set).
-template <typename T, int IN, int OUT, bool IO32 = false, bool SPARSE = false, bool PART = false>
+template <typename T, int IN, int OUT, int IO32 = 0, bool SPARSE = false, bool PART = false>
 __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs A, WK<T> wk, const T* __restrict__ xin,
                                                                       T* __restrict__ yout) {
     constexpr int RC = MvCfg<T>::RC, SX = 3 * RC, NCH = R / RC;
@@ -653,7 +654,7 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
                         for (int w = 0; w < RC; ++w) acc[w] += (double)vcomp(bv, w) * (double)vcomp(y, w);
                 }
             }
-            if constexpr (IO32) reinterpret_cast<float2*>(yout)[vi] = make_float2((float)y.x, (float)y.y);
+            if constexpr (IO32 == 1) reinterpret_cast<float2*>(yout)[vi] = make_float2((float)y.x, (float)y.y);
             else yv[vi] = y;
         }
         if constexpr (SUMS) {
@@ -918,7 +919,7 @@ __device__ inline double col_scale(const double* __restrict__ rr, int dom, int l
     return v > 0.0 ? 1.0 / sqrt(v) : 1.0;
 }
 
-// r32 = r64 (times its column scale when rr is given), x32 = 0 (residual replacement)
+// r32 = r64 (times its column scale when rr is given), x32 = 0 when given (residual replacement)
 __global__ __launch_bounds__(NT_UPD) void k_to32(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
                                                  const double* __restrict__ r64, float* __restrict__ r32,
                                                  float* __restrict__ x32, const double* __restrict__ rr) {
@@ -928,7 +929,7 @@ __global__ __launch_bounds__(NT_UPD) void k_to32(const BrickInfo* __restrict__ b
     const int64_t base = (int64_t)B.node_begin * 3 * R, cnt = (int64_t)(B.node_end - B.node_begin) * 3 * R;
     for (int64_t i = threadIdx.x; i < cnt; i += NT_UPD) {
         r32[base + i] = (float)(r64[base + i] * sc);
-        x32[base + i] = 0.f;
+        if (x32) x32[base + i] = 0.f;
     }
 }
 
@@ -961,15 +962,15 @@ constexpr double PIVOT_DROP = 1e-9;   // of P^T A P
 constexpr double ORTH_DROP = 1e-10;   // of P^T P
 
 // Gram partial sums over the rows of one slice of a domain's nodes:
-//   g[0] = A^T B, g[1] = A^T C (NGR = 2; zero for NGR = 1), FP64 sums of FP32
-// products.  Lane (i, j) of a wave keeps rows 3i..3i+2 x columns 3j..3j+2; the
+//   g[0] = A^T B, g[1] = A^T C (NGR = 2; zero for NGR = 1), FP64 sums of the
+// products of FP32 (or FP64) operands.  Lane (i, j) of a wave keeps rows 3i..3i+2 x columns 3j..3j+2; the
 // four waves take every fourth vector row.  The loads of the next GU rows of a
 // wave are issued before the products of the current ones (a slice is a long
 // sequential walk per wave); the waves' sums are added in wave order.
-template <int NGR>
+template <int NGR, typename TA = float, typename TB = float>
 __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices, const int* __restrict__ done,
-                                               const float* __restrict__ A, const float* __restrict__ B,
-                                               const float* __restrict__ C, double* __restrict__ gpart) {
+                                               const TA* __restrict__ A, const TB* __restrict__ B,
+                                               const TB* __restrict__ C, double* __restrict__ gpart) {
     __shared__ double s_g[NGR * RR];
     const int4 sl = slices[blockIdx.x];   // domain, first node, end node
     if (done[sl.x]) return;
@@ -984,7 +985,8 @@ __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices,
     const int64_t r1 = (int64_t)sl.z * 3;
     // two register sets: the next GU rows load into one while the products of the
     // other run; full steps (all GU rows in the slice) load without guards
-    float av[2][GU][3], bv[2][NGR][GU][3];
+    TA av[2][GU][3];
+    TB bv[2][NGR][GU][3];
     auto load = [&](int set, int64_t row0) {
 #pragma unroll
         for (int u = 0; u < GU; ++u) {
@@ -997,7 +999,7 @@ __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices,
             }
         }
     };
-    auto product = [&](const float (&a)[3], const float (&b)[3], double (&gg)[9]) {
+    auto product = [&](const TA (&a)[3], const TB (&b)[3], double (&gg)[9]) {
         double ad[3], bd[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -1028,7 +1030,8 @@ __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices,
         row0 += 4 * GU;
     }
     for (; row0 < r1; row0 += 4) {      // the last rows of the wave (fewer than GU)
-        float a[3], b[NGR][3];
+        TA a[3];
+        TB b[NGR][3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             a[i] = A[row0 * R + ci + i];
@@ -1213,6 +1216,44 @@ __global__ __launch_bounds__(NT_UPD) void k_bupdate(const BrickInfo* __restrict_
         double s = 0.0;
         for (int k = 0; k < NT_UPD / 6; ++k) s += s_a[k * 6 + qq][w];
         prr[(int64_t)blockIdx.x * R + tid] = s;
+    }
+}
+
+// The FP64 form of k_bupdate (block PCG on FP64 vectors): x += P alpha,
+// r -= Q alpha with x, r, Q in FP64 and the directions P in FP32, per-brick r.r.
+__global__ __launch_bounds__(NT_UPD) void k_bupdate64(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+                                                     const float* __restrict__ coef, double* __restrict__ x,
+                                                     double* __restrict__ r, const float* __restrict__ p,
+                                                     const double* __restrict__ q, double* __restrict__ prr) {
+    __shared__ float s_c[RR];
+    __shared__ double s_a[NT_UPD];
+    const BrickInfo B = bricks[blockIdx.x];
+    if (done[B.dom]) return;
+    const int tid = threadIdx.x, lc = tid % R, rw = tid / R;
+    for (int e = tid; e < RR; e += NT_UPD) s_c[e] = coef[(int64_t)B.dom * RR + e];
+    __syncthreads();
+    double acc = 0.0;
+    const int64_t r1 = (int64_t)B.node_end * 3;
+    for (int64_t row = (int64_t)B.node_begin * 3 + rw; row < r1; row += NT_UPD / R) {
+        const float* pr = p + row * R;
+        const double* qr = q + row * R;
+        double xv = x[row * R + lc], rv = r[row * R + lc];
+#pragma unroll 8
+        for (int k = 0; k < R; ++k) {
+            const double a = (double)s_c[k * R + lc];
+            xv += (double)pr[k] * a;
+            rv -= qr[k] * a;
+        }
+        x[row * R + lc] = xv;
+        r[row * R + lc] = rv;
+        acc += rv * rv;
+    }
+    s_a[tid] = acc;
+    __syncthreads();
+    if (tid < R) {
+        double sum = 0.0;
+        for (int k = 0; k < NT_UPD / R; ++k) sum += s_a[k * R + tid];
+        prr[(int64_t)blockIdx.x * R + tid] = sum;
     }
 }
 
@@ -2980,14 +3021,19 @@ int pcg_run(SolverBatch& b, const WK<T>& wk, T* x, T* r, T* p, T* q, const T* di
 // W = P^T Q resolves the directions of the block only as far as Q is accurate:
 // for smooth directions FP32 sums lose most digits of A P to cancellation, and
 // W's small eigenvalues (the ones that steer the block) drown in that error.
-int launch_mv_q64(SolverBatch& b, const float* p, float* q, const int* done) {
+// q64 set: Q is written as an FP64 vector (the block PCG on FP64 vectors)
+int launch_mv_q64(SolverBatch& b, const float* p, float* q, const int* done, double* q64 = nullptr) {
     MvArgs a{b.brick.as<BrickInfo>(), nullptr, 0, b.dom.as<DomInfo>(), b.halo.as<int>(), b.nmask.as<unsigned char>(),
              b.einfo.as<int>(), b.eE64.p, done, nullptr};
     int slot = -1;
     DTC_TRY(timed_begin(b, DTC_KSTAT_MV, &slot));
     const dim3 grid(b.n_bricks, chunk_split(b.n_bricks, R / MvCfg<double>::RC));
-    k_matvec<double, IN_V, OUT_MASKED, true><<<grid, MV_NT, 0, b.stream>>>(
-        a, cast_wk<double>(b.wk64), reinterpret_cast<const double*>(p), reinterpret_cast<double*>(q));
+    if (q64)
+        k_matvec<double, IN_V, OUT_MASKED, 2><<<grid, MV_NT, 0, b.stream>>>(
+            a, cast_wk<double>(b.wk64), reinterpret_cast<const double*>(p), q64);
+    else
+        k_matvec<double, IN_V, OUT_MASKED, 1><<<grid, MV_NT, 0, b.stream>>>(
+            a, cast_wk<double>(b.wk64), reinterpret_cast<const double*>(p), reinterpret_cast<double*>(q));
     DTC_HIP(hipGetLastError());
     DTC_TRY(timed_end(b, slot, 2, false, false, done));
     return DTC_OK;
@@ -3024,13 +3070,16 @@ int block_setup(SolverBatch& b) {
 }
 
 // per-domain Grams a^T b and a^T c (c may be null) into gsum
-int block_gram(SolverBatch& b, const float* a, const float* bb, const float* c) {
+template <typename T> struct NoDeduce { using type = T; };
+template <typename TA = float, typename TB = float>
+int block_gram(SolverBatch& b, const TA* a, const TB* bb, const typename NoDeduce<TB>::type* c) {
     hipStream_t s = b.stream;
     int* done = b.done.as<int>();
     if (b.n_slices > 0 && c)
-        k_gram<2><<<b.n_slices, GR_NT, 0, s>>>(b.slices.as<int4>(), done, a, bb, c, b.gpart.as<double>());
+        k_gram<2, TA, TB><<<b.n_slices, GR_NT, 0, s>>>(b.slices.as<int4>(), done, a, bb, c, b.gpart.as<double>());
     else if (b.n_slices > 0)
-        k_gram<1><<<b.n_slices, GR_NT, 0, s>>>(b.slices.as<int4>(), done, a, bb, nullptr, b.gpart.as<double>());
+        k_gram<1, TA, TB><<<b.n_slices, GR_NT, 0, s>>>(b.slices.as<int4>(), done, a, bb, nullptr,
+                                                       b.gpart.as<double>());
     k_gram_sum<<<dim3(b.n_dom, 2 * RR / 64), 256, 0, s>>>(b.dslice.as<int>(), done, b.gpart.as<double>(),
                                                          b.gsum.as<double>());
     DTC_HIP(hipGetLastError());
@@ -3051,10 +3100,42 @@ int block_orth(SolverBatch& b, const float* src, float* dst) {
     return DTC_OK;
 }
 
+// one block PCG iteration on FP64 vectors (x64, r64) with the FP32 multigrid as
+// preconditioner: Q = K P in FP64 (FP32 directions P, FP64 arithmetic), the
+// Grams in FP64, x += P alpha and r -= Q alpha in FP64; r is rounded to FP32
+// only as the V-cycle's input.  Used for high-contrast (greyscale) batches, whose
+// FP32 recursive residual loses the soft regions' part to the rounding of the
+// stiff regions' terms.
+int block_iteration64(SolverBatch& b, const WK<float>& wk, float* p, float* q, float* z, int64_t max_iters) {
+    hipStream_t s = b.stream;
+    const int D = b.n_dom, NBk = b.n_bricks;
+    int* done = b.done.as<int>();
+    const BrickInfo* bl = b.brick.as<BrickInfo>();
+    double* x = b.x64.as<double>();
+    double* r = b.r64.as<double>();
+    double* q64 = b.q64.as<double>();
+    if (b.micro20) DTC_TRY(mv20_q64(b, p, nullptr, done, q64));
+    else DTC_TRY(launch_mv_q64(b, p, nullptr, done, q64));
+    DTC_TRY((block_gram<float, double>(b, p, q64, r)));
+    k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 0, b.lfac.as<double>(), b.bcoef.as<float>());
+    k_bupdate64<<<NBk, NT_UPD, 0, s>>>(bl, done, b.bcoef.as<float>(), x, r, p, q64, b.part_b.as<double>());
+    k_bcheck<<<D, RED_NT, 0, s>>>(b.dom.as<DomInfo>(), done, b.part_b.as<double>(), b.thr2.as<double>(), nullptr,
+                                  b.iters.as<int64_t>(), max_iters);
+    k_to32<<<NBk, NT_UPD, 0, s>>>(bl, done, r, b.r32.as<float>(), nullptr, nullptr);
+    DTC_HIP(hipGetLastError());
+    DTC_TRY(mg_vcycle(b, wk, b.r32.as<float>(), z, nullptr));
+    DTC_TRY((block_gram<double, float>(b, q64, z, nullptr)));
+    k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 1, b.lfac.as<double>(), b.bcoef.as<float>());
+    k_bdir<<<NBk, NT_UPD, 0, s>>>(bl, done, b.bcoef.as<float>(), p, z, q);
+    DTC_HIP(hipGetLastError());
+    return block_orth(b, q, p);
+}
+
 // one block PCG iteration on (x, r); p is the direction block (orthonormal
 // columns), q and z are work vectors
 int block_iteration(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p, float* q, float* z,
                     int64_t max_iters) {
+    if (b.block64) return block_iteration64(b, wk, p, q, z, max_iters);
     hipStream_t s = b.stream;
     const int D = b.n_dom, NBk = b.n_bricks;
     int* done = b.done.as<int>();
@@ -3121,7 +3202,8 @@ int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p,
     }
     hipStream_t s = b.stream;
     const void* key[6] = {x, r, p, q, z, b.done.p};
-    bool same = b.block_graph.e && b.graph_iters == max_iters && b.graph_bricks == b.n_bricks;
+    bool same = b.block_graph.e && b.graph_iters == max_iters && b.graph_bricks == b.n_bricks &&
+                b.graph_mode == (int)b.block64;
     for (int k = 0; k < 6; ++k) same = same && b.graph_key[k] == key[k];
     if (!same) {
         b.block_graph.reset();
@@ -3144,6 +3226,7 @@ int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p,
         for (int k = 0; k < 6; ++k) b.graph_key[k] = key[k];
         b.graph_iters = max_iters;
         b.graph_bricks = b.n_bricks;
+        b.graph_mode = (int)b.block64;
     }
     // launch n copies the summary to host word n % 2 and records event n % 2
     hipEvent_t ev[2];
@@ -3182,7 +3265,7 @@ int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p,
 // then per load case activity and targets (k_red_outer)
 int residual64(SolverBatch& b, const WK<double>& wk, const double* x64, int first, const dtc_solve_opts& o) {
     DTC_TRY((launch_mv<double, IN_U, OUT_RESID>(b, wk, x64, b.r64.as<double>(), b.part_c.as<double>(), nullptr)));
-    const double* itol2 = b.precision == DTC_FP32_REFINE ? b.itol2.as<double>() : nullptr;
+    const double* itol2 = b.precision == DTC_FP32_REFINE && !b.block64 ? b.itol2.as<double>() : nullptr;
     k_red_outer<<<b.n_dom, RED_NT, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_c.as<double>(), first,
                                                   o.rel_tol * o.rel_tol, itol2, b.rr0.as<double>(), b.rr.as<double>(),
                                                   b.active64.as<int>(), b.active.as<int>(), b.thr2.as<double>(),
@@ -3422,6 +3505,15 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
         b.n_slots = 0;
     }
     std::vector<int64_t> nouter(D, 0);
+    // high-contrast batches (solid moduli differing by more than 100x): the block
+    // iterates on FP64 vectors with the FP32 V-cycle (DTC_BLOCK64=1 / 0 overrides)
+    {
+        double cmax = 1.0;
+        for (double c : b.h_contrast) cmax = std::fmax(cmax, c);
+        const char* e64 = std::getenv("DTC_BLOCK64");
+        b.block64 = block && (e64 ? e64[0] == '1' : cmax > 100.0);
+        if (b.block64) DTC_TRY(b.q64.ensure((size_t)nv * 8));
+    }
     DTC_TRY(residual64(b, wk64, nullptr, 1, o));          // r64 = f (lifted right-hand side)
 
     if (b.precision == DTC_FP64) {
@@ -3469,6 +3561,7 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
         // a block phase ends at its targets; the cap only bounds FP32 drift
         const char* brp = std::getenv("DTC_BLOCK_REPLACE");
         const int64_t block_replace = brp ? std::max(1, std::atoi(brp)) : 400;
+        const int64_t block_replace64 = brp ? block_replace : 1000;   // FP64 recursion drifts far less
         float* bp = p32;
         float* bq = b.q32.as<float>();
         float* bz = z32;
@@ -3483,6 +3576,17 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
             // directions mix all 24 load cases, whose residuals drift apart in size
             k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.r64.as<double>(), r32, x32,
                                                  block ? b.rr.as<double>() : nullptr);
+            if (block && b.block64) {
+                // FP64 vectors: the block restarts from M r at every true-residual check
+                k_to32<<<b.n_bricks, NT_UPD, 0, s>>>(bl, b.done64.as<int>(), b.r64.as<double>(), r32, nullptr, nullptr);
+                DTC_HIP(hipGetLastError());
+                DTC_TRY(mg_vcycle(b, wk32, r32, bz, nullptr));
+                DTC_TRY(block_orth(b, bz, bp));
+                first = false;
+                DTC_TRY(block_run(b, wk32, x32, r32, bp, bq, bz, cap, block_replace64));
+                DTC_TRY(residual64(b, wk64, b.x64.as<double>(), 0, o));
+                continue;
+            }
             if (block) {
                 // the direction block restarts from M r at every residual replacement
                 // (DTC_BLOCK_RESTART=0: it is kept, spanning the same space after the

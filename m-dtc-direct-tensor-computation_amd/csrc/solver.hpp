@@ -122,6 +122,7 @@ struct SolverBatch {
     DevMem lfac;                   // double [dom][24 x 24]: Cholesky factor of P^T A P
     DevMem bcoef;                  // float [dom][24 x 24]: alpha, then beta, of the current step
     bool block_q64 = false;        // Q = A P in FP64 arithmetic
+    bool block64 = false;          // block PCG on FP64 vectors (x, r, Q) with the FP32 V-cycle
     int precond = 0;               // DTC_PC_JACOBI / DTC_PC_MG
     int mg_degree = 2;             // Chebyshev degree of the fine and intermediate levels
     double cheb_lmax = 0.0;        // element-by-element bound of lambda_max(D^-1 K)
@@ -174,6 +175,7 @@ struct SolverBatch {
     const void* graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int64_t graph_iters = -1;
     int graph_bricks = -1;
+    int graph_mode = -1;           // block64 of the captured iterations
     DevMem all_done_dev;           // int: 1 when every domain's done flag is set
     HostFlags all_done_host;       // int [2]: copies of all_done_dev, one per launch parity
     int graph_launches = 0;        // graph launches of the last solve (statistics)
@@ -216,8 +218,9 @@ int batch_add_domain20(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& 
                        double escale, int lc_base, hipStream_t s);
 int finalize20(SolverBatch& b);
 int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout, double* part, const int* done);
-// Q = K P of the block PCG on a HEX20 batch: FP32 vectors, FP64 arithmetic (masked rows)
-int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done);
+// Q = K P of the block PCG on a HEX20 batch: FP32 directions, FP64 arithmetic
+// (masked rows); Q is written to q (FP32), or to q64 (FP64) when that is set
+int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done, double* q64 = nullptr);
 int project20(SolverBatch& b);
 int fields20(SolverBatch& b, int i, int ncol, double* u_dev, double* r_dev);
 // sensitivity dL/dE per voxel of domain i.

@@ -403,13 +403,14 @@ __global__ __launch_bounds__(256) void k20_elem_mf(Dom20Dev G, int dom, const in
 
 // y on the pseudo-bricks: the sum of the adjacent elements' forces; MASKED zeroes
 // Dirichlet rows, MASKED_DOT also writes per-brick sums of x . y
-// (T: the type of the element forces and of the sums; the vectors are FP32)
-template <typename T, int OUT>
+// (T: the type of the element forces and of the sums; x is an FP32 vector, y one
+// of type TO)
+template <typename T, int OUT, typename TO = float>
 __global__ __launch_bounds__(NT20) void k20_gather(const BrickInfo* __restrict__ bricks,
                                                   const Dom20Dev* __restrict__ d20,
                                                   const int64_t* __restrict__ ebase, const int* __restrict__ done,
                                                   const T* __restrict__ ye, const float* __restrict__ xin,
-                                                  float* __restrict__ yout, double* __restrict__ partial) {
+                                                  TO* __restrict__ yout, double* __restrict__ partial) {
     __shared__ double s_red[NT20];
     const BrickInfo B = bricks[blockIdx.x];
     if (done && done[B.dom]) return;
@@ -430,7 +431,7 @@ __global__ __launch_bounds__(NT20) void k20_gather(const BrickInfo* __restrict__
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int64_t o = ((int64_t)q * 3 + d) * R + r;
-            const float v = bc ? 0.f : (float)y[d];
+            const TO v = bc ? TO(0) : (TO)y[d];
             if (OUT == 0) acc += (double)xin[o] * (double)v;
             yout[o] = v;
         }
@@ -501,12 +502,13 @@ int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout
     return set_error(DTC_ERR_UNSUPPORTED, "HEX20 micro operator: mode (%d, %d)", in, out);
 }
 
-int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done) {
+int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done, double* q64) {
     if (b.n_bricks == 0) return DTC_OK;
     hipStream_t s = b.stream;
     const char* ee = std::getenv("DTC_HEX20_GATHER");
     if (ee && ee[0] == '1') {   // the node-gather form
         Mv20 a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.d20dev.as<Dom20Dev>(), done, nullptr};
+        if (q64) return set_error(DTC_ERR_UNSUPPORTED, "HEX20 node-gather operator with an FP64 output");
         k_matvec20<double, 0, 3, float><<<b.n_bricks, NT20, 0, s>>>(a, b.k20_64.as<double>(), p, q);
         DTC_HIP(hipGetLastError());
         return DTC_OK;
@@ -518,9 +520,14 @@ int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done) {
         k20_elem_mf<double><<<nb(ne, EmCfg<double>::EM), 256, 0, s>>>(G, g, done, b.k20_64.as<double>(), p,
                                                                       b.ye20.as<double>() + b.h_ebase20[g] * 60 * R, ne);
     }
-    k20_gather<double, 3><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
-                                                      b.ebase20.as<int64_t>(), done, b.ye20.as<double>(), p, q,
-                                                      nullptr);
+    if (q64)
+        k20_gather<double, 3, double><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
+                                                                  b.ebase20.as<int64_t>(), done, b.ye20.as<double>(),
+                                                                  p, q64, nullptr);
+    else
+        k20_gather<double, 3><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
+                                                          b.ebase20.as<int64_t>(), done, b.ye20.as<double>(), p, q,
+                                                          nullptr);
     DTC_HIP(hipGetLastError());
     return DTC_OK;
 }
