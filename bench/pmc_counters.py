@@ -6,7 +6,9 @@
 Runs one rocprofv3 --pmc pass (the counters of one pass must fit the hardware's
 per-block limits: at most 8 SQ counters) over one bench.py step of the workload
 (as a child process, the program directly after `--`), and averages every counter
-over the fine-level dispatches (largest grid) of each timed kernel: the CG matvec
+over the fine-level dispatches (the most dispatched grid, the larger on a tie: the
+coarse levels run smaller grids, the launch-shape autotuner a few split ones) of
+each timed kernel: the CG matvec
 Q = K P (mv), the fused Chebyshev smoothing step (cheb), the restriction
 residual (bres) and the FP64 residual (resid).  Prints one JSON object.
 """
@@ -62,7 +64,12 @@ def main() -> None:
         entry = {"kernel": label, "variants": {}}
         for name, rs in variants.items():
             if all(r[1] is not None for r in rs):
-                g = max(r[1] for r in rs)
+                # the solve's fine-level grid: the most dispatched grid, the larger on a tie
+                # (the launch-shape autotuner runs a few launches on split grids first)
+                cnt = {}
+                for r in rs:
+                    cnt.setdefault(r[1], set()).add(r[2])
+                g = max(cnt, key=lambda k: (len(cnt[k]), k))
                 rs = [r for r in rs if r[1] == g]
             per = {}
             for _, _, disp, cn, v in rs:

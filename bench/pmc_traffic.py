@@ -10,8 +10,9 @@ Runs (as child processes, the program directly after `--`):
 FETCH_SIZE / WRITE_SIZE are reported in KiB per dispatch.  The copy reads and
 writes exactly 1 GiB, so `factor = 1 GiB / counter(copy)` converts each counter
 into bytes; traffic = FETCH * f_fetch + WRITE * f_write per dispatch, averaged
-over the fine-level dispatches of the kernel (those with the largest grid: the
-coarse multigrid levels launch the same kernel on smaller grids), the launches
+over the fine-level dispatches of the kernel (the most dispatched grid, the
+larger on a tie: the coarse multigrid levels launch the same kernel on smaller
+grids, the launch-shape autotuner a few launches on split grids), the launches
 whose HIP-event average and algorithmic bytes bench.py reports.
 Writes profiles/traffic_<kernel>.json, which bench.py reports as the kernel's
 traffic (and as roofline.traffic for the dominant kernel) when its batch and
@@ -58,7 +59,7 @@ def _rows(dbdir: str, counter: str):
 
 def _counter(dbdir: str, counter: str, kname: str) -> tuple[float, int]:
     """(mean value per fine-level call, dispatches): for every kernel variant whose
-    name starts with kname, the mean over its largest-grid dispatches; the variants'
+    name starts with kname, the mean over its fine-level dispatches; the variants'
     means are added (one dispatch of each per call)."""
     rows = [r for r in _rows(dbdir, counter) if kname in r[0].replace("(anonymous namespace)::", "")]
     if not rows:
@@ -69,7 +70,10 @@ def _counter(dbdir: str, counter: str, kname: str) -> tuple[float, int]:
     total, n = 0.0, 0
     for name, rs in by.items():
         if all(r[1] is not None for r in rs):
-            g = max(r[1] for r in rs)
+            cnt = {}
+            for r in rs:
+                cnt[r[1]] = cnt.get(r[1], 0) + 1
+            g = max(cnt, key=lambda k: (cnt[k], k))   # most dispatched grid, the larger on a tie
             rs = [r for r in rs if r[1] == g]
         total += sum(r[2] for r in rs) / len(rs)
         n = max(n, len(rs))

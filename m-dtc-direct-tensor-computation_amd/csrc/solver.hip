@@ -3561,7 +3561,10 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
         // a block phase ends at its targets; the cap only bounds FP32 drift
         const char* brp = std::getenv("DTC_BLOCK_REPLACE");
         const int64_t block_replace = brp ? std::max(1, std::atoi(brp)) : 400;
-        const int64_t block_replace64 = brp ? block_replace : 1000;   // FP64 recursion drifts far less
+        // FP64 vectors: the recursion drifts far less, and a restart discards the
+        // block's Krylov space, so a phase runs to its target (the true residual
+        // is checked then)
+        const int64_t block_replace64 = brp ? block_replace : 20000;
         float* bp = p32;
         float* bq = b.q32.as<float>();
         float* bz = z32;
