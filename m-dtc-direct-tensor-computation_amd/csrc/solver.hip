@@ -1221,39 +1221,61 @@ __global__ __launch_bounds__(NT_UPD) void k_bupdate(const BrickInfo* __restrict_
 
 // The FP64 form of k_bupdate (block PCG on FP64 vectors): x += P alpha,
 // r -= Q alpha with x, r, Q in FP64 and the directions P in FP32, per-brick r.r.
+// Of the 6 threads of a row, thread q keeps load cases 4q..4q+3 (as k_bupdate).
 __global__ __launch_bounds__(NT_UPD) void k_bupdate64(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
                                                      const float* __restrict__ coef, double* __restrict__ x,
                                                      double* __restrict__ r, const float* __restrict__ p,
                                                      const double* __restrict__ q, double* __restrict__ prr) {
-    __shared__ float s_c[RR];
-    __shared__ double s_a[NT_UPD];
+    __shared__ float4 s_c[R * 6];
+    __shared__ double s_a[NT_UPD][4];
     const BrickInfo B = bricks[blockIdx.x];
     if (done[B.dom]) return;
-    const int tid = threadIdx.x, lc = tid % R, rw = tid / R;
-    for (int e = tid; e < RR; e += NT_UPD) s_c[e] = coef[(int64_t)B.dom * RR + e];
+    const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
+    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)B.dom * RR);
+    for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
     __syncthreads();
-    double acc = 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int64_t r1 = (int64_t)B.node_end * 3;
-    for (int64_t row = (int64_t)B.node_begin * 3 + rw; row < r1; row += NT_UPD / R) {
-        const float* pr = p + row * R;
-        const double* qr = q + row * R;
-        double xv = x[row * R + lc], rv = r[row * R + lc];
-#pragma unroll 8
-        for (int k = 0; k < R; ++k) {
-            const double a = (double)s_c[k * R + lc];
-            xv += (double)pr[k] * a;
-            rv -= qr[k] * a;
+    for (int64_t row = (int64_t)B.node_begin * 3 + rw; row < r1; row += NT_UPD / 6) {
+        const float4* pr = reinterpret_cast<const float4*>(p + row * R);
+        const double2* qr = reinterpret_cast<const double2*>(q + row * R);
+        double2* xr = reinterpret_cast<double2*>(x + row * R) + 2 * qd;
+        double2* rr = reinterpret_cast<double2*>(r + row * R) + 2 * qd;
+        double xv[4], rv[4];
+        {
+            const double2 x0 = xr[0], x1 = xr[1], r0 = rr[0], r1v = rr[1];
+            xv[0] = x0.x; xv[1] = x0.y; xv[2] = x1.x; xv[3] = x1.y;
+            rv[0] = r0.x; rv[1] = r0.y; rv[2] = r1v.x; rv[3] = r1v.y;
         }
-        x[row * R + lc] = xv;
-        r[row * R + lc] = rv;
-        acc += rv * rv;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const float4 pv = pr[c];
+            const double2 qa = qr[2 * c], qb = qr[2 * c + 1];
+            const double qk[4] = {qa.x, qa.y, qb.x, qb.y};
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const float4 a = s_c[(4 * c + w) * 6 + qd];
+                const double pk = (double)vcomp(pv, w);
+                xv[0] += pk * (double)a.x; xv[1] += pk * (double)a.y; xv[2] += pk * (double)a.z; xv[3] += pk * (double)a.w;
+                rv[0] -= qk[w] * (double)a.x; rv[1] -= qk[w] * (double)a.y;
+                rv[2] -= qk[w] * (double)a.z; rv[3] -= qk[w] * (double)a.w;
+            }
+        }
+        xr[0] = make_double2(xv[0], xv[1]);
+        xr[1] = make_double2(xv[2], xv[3]);
+        rr[0] = make_double2(rv[0], rv[1]);
+        rr[1] = make_double2(rv[2], rv[3]);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) acc[w] += rv[w] * rv[w];
     }
-    s_a[tid] = acc;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s_a[tid][w] = acc[w];
     __syncthreads();
     if (tid < R) {
-        double sum = 0.0;
-        for (int k = 0; k < NT_UPD / R; ++k) sum += s_a[k * R + tid];
-        prr[(int64_t)blockIdx.x * R + tid] = sum;
+        const int qq = tid / 4, w = tid % 4;
+        double s = 0.0;
+        for (int k = 0; k < NT_UPD / 6; ++k) s += s_a[k * 6 + qq][w];
+        prr[(int64_t)blockIdx.x * R + tid] = s;
     }
 }
 

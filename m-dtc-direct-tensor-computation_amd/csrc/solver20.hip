@@ -420,12 +420,30 @@ __global__ __launch_bounds__(NT20) void k20_gather(const BrickInfo* __restrict__
     double acc = 0.0;
     for (int q = B.node_begin + g; q < B.node_end; q += NT20 / R) {
         const int ql = (int)(q - G.node_begin);
+        // the node's (element, local node) entries in groups of 8, all loads of a
+        // group issued before its sums (list order kept; absent entries add 0)
         T y[3] = {T(0), T(0), T(0)};
-        for (int k = G.adj_off[ql]; k < G.adj_off[ql + 1]; ++k) {
-            const int e = G.adj[k] >> 5, a = G.adj[k] & 31;
-            const T* src = ye + ((eb + e) * 60 + 3 * a) * R + r;
+        const int k1 = G.adj_off[ql + 1];
+        for (int kb = G.adj_off[ql]; kb < k1; kb += 8) {
+            int ent[8];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) y[d] += src[d * R];
+            for (int u = 0; u < 8; ++u) ent[u] = kb + u < k1 ? G.adj[kb + u] : -1;
+            T v[8][3];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (ent[u] >= 0) {
+                    const T* src = ye + ((eb + (ent[u] >> 5)) * 60 + 3 * (ent[u] & 31)) * R + r;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) v[u][d] = src[d * R];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) v[u][d] = T(0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) y[d] += v[u][d];
         }
         const bool bc = G.bcf[ql] != 0;
 #pragma unroll

@@ -9,7 +9,7 @@ can afford it, checked through size-independent properties and the oracle:
    5 degree rotation grid): every domain converges, the batch equals the single
    solves, the two smallest match the oracle's fixtures;
 4. greyscale HEX20 micro + HEX20 macro with per-voxel modulus: vs the oracle (12^3);
-   at 48^3 the mixed p-multigrid solve converges without FP64 fallback, and the
+   at 96^3 the mixed p-multigrid solve converges without FP64 fallback, and the
    60 x 60 macro stiffness is symmetric with the rigid translations in its null space;
 5. greyscale gradient: central differences at 32^3 (mixed solve to rel-res 1e-11), and
    the sensitivity of a 128^3 domain costs far less than its forward solve."""
@@ -140,14 +140,14 @@ def test_config4_greyscale_hex20_micro_and_macro():
                        emod=modulus[ijk[:, 2], ijk[:, 1], ijk[:, 0]])
     KM, Cm, EE = O.effective(o["ff"], dim, macro_order=2)
     assert _rel(r["ff"], o["ff"]) < 1e-5 and _rel(r["C"], Cm) < 1e-5 and _rel(r["K_macro"], KM) < 1e-4
-    # 48^3 (BV/TV 0.20, hinged clusters included): convergence of the mixed solve
-    # without an FP64 fallback, and the properties of the 60 x 60 macro stiffness
-    # (symmetry, rigid translations)
-    grey, modulus = synth.greyscale(48, seed=4, bvtv=0.20)
+    # 96^3 (the configuration's size, BV/TV 0.20, hinged clusters included): convergence
+    # of the mixed solve without an FP64 fallback, and the properties of the 60 x 60
+    # macro stiffness (symmetry, rigid translations)
+    grey, modulus = synth.greyscale(96, seed=4, bvtv=0.20)
     d = dtc.Domain(grey, llimit=1, delta=(0.01,) * 3, modulus=modulus, E=10000.0, micro_nen=20, macro_order=2)
     t0 = time.perf_counter()
     r = dtc.solve_domain(d, precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
-    print(f"config 4 at 48^3: {r['n_elems']} elements, {r['iterations']} iterations, "
+    print(f"config 4 at 96^3: {r['n_elems']} elements, {r['iterations']} iterations, "
           f"{r['outer_iterations']} replacements, {time.perf_counter() - t0:.1f} s")
     assert r["status"] == 0 and r["relres"] <= 1e-8
     assert r["precision_used"] == dtc.DTC_FP32_REFINE
