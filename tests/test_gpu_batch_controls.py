@@ -199,3 +199,24 @@ def test_sparse_brick_kernel_equals_dense(monkeypatch):
             for k in ("C", "K_macro", "ff"):
                 ref = np.abs(a[k]).max()
                 assert np.abs(a[k] - c[k]).max() <= 1e-7 * ref, (pc, k)
+
+
+def test_whole_segment_output_stores_equal_per_chunk_stores(monkeypatch):
+    """The dense FP32 Chebyshev step keeps each item's outputs of all load-case
+    chunks in registers and stores them after the last chunk; the values are the
+    same as with one store per chunk (DTC_MV_DEFER=0), so the solves agree bit for
+    bit (multigrid block PCG and Jacobi PCG, fresh handles with the same launch
+    shapes)."""
+    dtc = _dtc()
+    doms = [_binary(dtc, 24, 71), _grey(dtc, 20, 72)]
+    for pc in (dtc.DTC_PC_MG, dtc.DTC_PC_JACOBI):
+        kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=pc)
+        monkeypatch.setenv("DTC_MV_DEFER", "0")
+        ra = dtc.solve_batch(doms, **kw)
+        monkeypatch.delenv("DTC_MV_DEFER")
+        rb = dtc.solve_batch(doms, **kw)
+        for a, c in zip(ra, rb):
+            assert a["status"] == c["status"] == 0
+            assert a["iterations"] == c["iterations"]
+            for k in ("C", "K_macro", "ff"):
+                assert np.array_equal(a[k], c[k]), (pc, k)
