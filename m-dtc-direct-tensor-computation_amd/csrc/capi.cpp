@@ -761,7 +761,6 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
     if (i < 0 || i >= (int64_t)b->d.size()) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %lld", (long long)i);
     const dtc_domain_desc& d = b->d[i];
     if (d.macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements the HEX08 macro element");
-    if (d.micro_nen != 8) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements HEX08 micro elements");
     const int gi = b->g0[i];
     SolverBatch& sb = b->subs[b->sbi[i]]->sb;
     const size_t nv = (size_t)d.n[0] * d.n[1] * d.n[2];

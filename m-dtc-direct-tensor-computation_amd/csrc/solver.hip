@@ -3757,7 +3757,7 @@ int batch_fields(SolverBatch& b, int i, int ncol, double* u_out, double* r_out) 
 
 int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host) {
     if (i < 0 || i >= b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %d", i);
-    if (b.micro20) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements HEX08 micro elements");
+    if (b.micro20) return grad20(b, i, M, dL_dE_host);
     const DomInfo& D = b.h_dom[i];
     const int64_t nvox = (int64_t)D.nx * D.ny * D.nz;
     const int64_t nn = b.n_nodes > 0 ? b.n_nodes : 1;

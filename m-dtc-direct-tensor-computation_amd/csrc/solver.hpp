@@ -28,6 +28,8 @@ struct Dom20 {
     DevMem coords;                 // double [node][3], mesh order
     DevMem elems;                  // int [elem][20]: 0-based local node ids
     DevMem emod;                   // double per element
+    DevMem evox;                   // int per element: its voxel (x fastest) in the domain
+    int64_t nvox = 0;
     DevMem adj_off, adj;           // node -> sorted (element << 5 | local node)
     DevMem bcf;                    // uint8 per node: Dirichlet node
     DevMem scan_tmp;
@@ -223,6 +225,8 @@ int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout
 int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done, double* q64 = nullptr);
 int project20(SolverBatch& b);
 int fields20(SolverBatch& b, int i, int ncol, double* u_dev, double* r_dev);
+// dL/dE per voxel of domain group i of a HEX20 batch (HEX08 macro: M is 24 x 6)
+int grad20(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host);
 // sensitivity dL/dE per voxel of domain i.
 int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host);
 

@@ -29,11 +29,13 @@ constexpr int PB = 128;          // nodes per pseudo-brick
 constexpr int FFD = 60;          // macro DOF slots of the force partials
 
 // element -> voxel modulus (kept elements only)
+// (and element -> voxel index, the sensitivity's output position)
 __global__ void k20_emod(const int* __restrict__ keepE, const int* __restrict__ newE, const double* __restrict__ modulus,
-                         double E, double escale, int64_t nvox, double* __restrict__ emod) {
+                         double E, double escale, int64_t nvox, double* __restrict__ emod, int* __restrict__ evox) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvox || !keepE[v]) return;
     emod[newE[v]] = (modulus ? modulus[v] : E) * escale;
+    evox[newE[v]] = (int)v;
 }
 
 // 1-based int64 connectivity -> 0-based int32 local node ids (-1: node dropped)
@@ -576,9 +578,11 @@ int batch_add_domain20(SolverBatch& b, const MeshDev& m, const dtc_domain_desc& 
     DTC_TRY(G.elems.alloc((size_t)ne * 20 * 4));
     k20_elems<<<nb(ne * 20, 256), 256, 0, s>>>(e64.as<int64_t>(), ne * 20, G.elems.as<int>());
     DTC_TRY(G.emod.alloc((size_t)ne * 8));
+    DTC_TRY(G.evox.alloc((size_t)ne * 4));
     const int64_t nvox = (int64_t)m.nx * m.ny * m.nz;
+    G.nvox = nvox;
     k20_emod<<<nb(nvox, 256), 256, 0, s>>>(m.keepE.as<int>(), m.newE.as<int>(), modulus_dev, d.E, escale, nvox,
-                                          G.emod.as<double>());
+                                          G.emod.as<double>(), G.evox.as<int>());
     // boundary nodes (fewer than 6 constrained DOFs fails the domain, f-src/mod_struct_preprocess.f90:471-473)
     DTC_TRY(G.bcf.alloc((size_t)nn));
     DTC_TRY(dimd.alloc(3 * 8));
@@ -694,6 +698,89 @@ int project20(SolverBatch& b) {
                                                        b.d20dev.as<Dom20Dev>(), b.r64.as<double>(),
                                                        b.part_ff.as<double>());
     DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+// ---------------------------------------------------------------- sensitivity (HEX20 micro, HEX08 macro)
+// h[q][j][d] = sum_lc M[lc][j] u_lc(q, d), u = x + x_bc (the HEX08 form in k_grad_h)
+__global__ __launch_bounds__(256) void k20_grad_h(Dom20Dev G, DomInfo D, const double* __restrict__ x,
+                                                  const double* __restrict__ Mj, double* __restrict__ h) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)G.n_nodes * 18) return;
+    const int q = (int)(i / 18), j = (int)(i % 18) / 3, d = (int)(i % 3);
+    const bool bc = G.bcf[q] != 0;
+    double s = 0.0;
+    for (int lc = 0; lc < R; ++lc) {
+        const double u = bc ? bc20(D, G, q, d, lc) : x[((G.node_begin + q) * 3 + d) * R + lc];
+        s += Mj[lc * 6 + j] * u;
+    }
+    h[i] = s;
+}
+
+// dL/dE_v = sum_j h_j,e^T Khat u_j,e per element (one wave per element: lane r < 60
+// forms row r of Khat u_j, the wave sums h_j . (Khat u_j) in a fixed order), written
+// at the element's voxel
+constexpr int G20_EPB = 4;   // elements (waves) per workgroup
+__global__ __launch_bounds__(64 * G20_EPB) void k20_grad_elem(Dom20Dev G, DomInfo D, const double* __restrict__ Kg,
+                                                              const double* __restrict__ x,
+                                                              const double* __restrict__ h,
+                                                              const int* __restrict__ evox, int64_t ne,
+                                                              double* __restrict__ g) {
+    __shared__ double sK[3600];
+    __shared__ double su[G20_EPB][6][60];
+    __shared__ double sh[G20_EPB][6][60];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < 3600; i += 64 * G20_EPB) sK[i] = Kg[i];
+    const int64_t e = (int64_t)blockIdx.x * G20_EPB + w;
+    if (e < ne) {
+        for (int i = lane; i < 6 * 60; i += 64) {
+            const int j = i / 60, k = i % 60, a = k / 3, d = k % 3;
+            const int q = G.elems[e * 20 + a];
+            double u = 0.0, hv = 0.0;
+            if (q >= 0) {
+                u = G.bcf[q] ? bc20(D, G, q, d, j) : x[((G.node_begin + q) * 3 + d) * R + j];
+                hv = h[((int64_t)q * 6 + j) * 3 + d];
+            }
+            su[w][j][k] = u;
+            sh[w][j][k] = hv;
+        }
+    }
+    __syncthreads();
+    if (e >= ne) return;
+    double part = 0.0;
+    if (lane < 60) {
+        for (int j = 0; j < 6; ++j) {
+            double ku = 0.0;
+            for (int c = 0; c < 60; ++c) ku += sK[lane * 60 + c] * su[w][j][c];
+            part += sh[w][j][lane] * ku;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if (lane == 0) g[evox[e]] = part;
+}
+
+int grad20(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host) {
+    if (i < 0 || i >= b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %d", i);
+    const Dom20& G = b.d20[i];
+    const Dom20Dev& Gd = b.h_d20dev[i];
+    const DomInfo& D = b.h_dom[i];
+    hipStream_t s = b.stream;
+    const int64_t nvox = G.nvox, ne = b.h_elems[i];
+    DevMem h, Md, g;
+    DTC_TRY(Md.alloc(24 * 6 * 8));
+    DTC_TRY(g.alloc((size_t)std::max<int64_t>(nvox, 1) * 8));
+    DTC_HIP(hipMemcpyAsync(Md.p, M, 24 * 6 * 8, hipMemcpyHostToDevice, s));
+    DTC_HIP(hipMemsetAsync(g.p, 0, (size_t)std::max<int64_t>(nvox, 1) * 8, s));   // voxels without an element: 0
+    if (ne > 0 && Gd.n_nodes > 0) {
+        DTC_TRY(h.alloc((size_t)Gd.n_nodes * 18 * 8));
+        k20_grad_h<<<nb((int64_t)Gd.n_nodes * 18, 256), 256, 0, s>>>(Gd, D, b.x64.as<double>(), Md.as<double>(),
+                                                                     h.as<double>());
+        k20_grad_elem<<<nb(ne, G20_EPB), 64 * G20_EPB, 0, s>>>(Gd, D, b.k20_64.as<double>(), b.x64.as<double>(),
+                                                               h.as<double>(), G.evox.as<int>(), ne, g.as<double>());
+    }
+    DTC_HIP(hipGetLastError());
+    DTC_HIP(hipMemcpyAsync(dL_dE_host, g.p, (size_t)nvox * 8, hipMemcpyDeviceToHost, s));
+    DTC_HIP(hipStreamSynchronize(s));
     return DTC_OK;
 }
 

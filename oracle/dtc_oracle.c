@@ -1038,15 +1038,18 @@ int dtco_rotation_search(const double EE[36], int criterion, double step_deg, do
  * convergence this equals w_a^T K u_b with w_a = sum_b u_b vv_eps^-1(b,a) the
  * harmonic extension of g_a (vv_eps: load-case matrix at eps = 1e-6, the
  * value init_displ_hexe8 uses).  Hence dC_ij/dE_e = meps_j sum_a W_ia
- * w_a,e^T Khat u_j,e and dL/dE_e = sum_j h_j,e^T Khat u_j,e. */
+ * w_a,e^T Khat u_j,e and dL/dE_e = sum_j h_j,e^T Khat u_j,e, for HEX08 and HEX20
+ * micro elements (nen nodes, Khat of the element type) under the HEX08 macro
+ * element's 24 load cases. */
 int dtco_grad_modulus(const double* nodes, int64_t n_nodes, const int64_t* elems,
-                      int64_t n_elems, double nu, double a, const double min_c[3],
+                      int64_t n_elems, int nen, double nu, double a, const double min_c[3],
                       const double max_c[3], const double x_D_phy[3], double rve_strain,
                       const double* u, const double G[36], double* dL_dE) {
     (void)nodes; (void)min_c; (void)max_c; (void)rve_strain;
-    const int R = 24;
+    if (nen != 8 && nen != 20) return 5;
+    const int R = 24, nd = 3 * nen;
     const int64_t m = 3 * n_nodes;
-    double vv[576], W[6 * 24], M[24 * 6], Khat[576];
+    double vv[576], W[6 * 24], M[24 * 6], Khat[3600];
     dtco_vv_lin(1.e-6, vv);
     if (invert(vv, R)) return 4;
     fv_weights_lin(x_D_phy, W);
@@ -1061,28 +1064,28 @@ int dtco_grad_modulus(const double* nodes, int64_t n_nodes, const int64_t* elems
             }
             M[b + R * j] = s * MEPS[j];
         }
-    dtco_ke(8, 1.0, nu, a, Khat);
+    dtco_ke(nen, 1.0, nu, a, Khat);
     for (int64_t e = 0; e < n_elems; ++e) {
-        const int64_t* en = elems + 8 * e;
-        double ue[24][24];   /* [lc][local dof] */
+        const int64_t* en = elems + (int64_t)nen * e;
+        double ue[24][60];   /* [lc][local dof] */
         for (int lc = 0; lc < R; ++lc)
-            for (int l = 0; l < 8; ++l)
+            for (int l = 0; l < nen; ++l)
                 for (int d = 0; d < 3; ++d)
                     ue[lc][3 * l + d] = en[l] > 0 ? u[(size_t)m * lc + 3 * (en[l] - 1) + d] : 0.0;
         double g = 0.0;
         for (int j = 0; j < 6; ++j) {
-            double h[24], ku[24];
-            for (int t = 0; t < 24; ++t) {
+            double h[60], ku[60];
+            for (int t = 0; t < nd; ++t) {
                 double s = 0.0;
                 for (int b = 0; b < R; ++b) s += M[b + R * j] * ue[b][t];
                 h[t] = s;
             }
-            for (int r = 0; r < 24; ++r) {
+            for (int r = 0; r < nd; ++r) {
                 double s = 0.0;
-                for (int c = 0; c < 24; ++c) s += Khat[r + 24 * c] * ue[j][c];
+                for (int c = 0; c < nd; ++c) s += Khat[r + nd * c] * ue[j][c];
                 ku[r] = s;
             }
-            for (int t = 0; t < 24; ++t) g += h[t] * ku[t];
+            for (int t = 0; t < nd; ++t) g += h[t] * ku[t];
         }
         dL_dE[e] = g;
     }

@@ -211,7 +211,8 @@ def grad_modulus(nodes, elems, nu, a, min_c, max_c, x_D_phy, u, G, rve_strain=1e
     Gc = np.ascontiguousarray(np.asarray(G, np.float64).T)
     out = np.zeros(elems.shape[0])
     _chk("grad_modulus", lib().dtco_grad_modulus(
-        _p(nodes), C.c_int64(nodes.shape[0]), _p(elems), C.c_int64(elems.shape[0]), C.c_double(nu),
+        _p(nodes), C.c_int64(nodes.shape[0]), _p(elems), C.c_int64(elems.shape[0]), C.c_int(elems.shape[1]),
+        C.c_double(nu),
         C.c_double(a), _p(_f64(*min_c)), _p(_f64(*max_c)), _p(_f64(*x_D_phy)), C.c_double(rve_strain),
         _p(uc), _p(Gc), _p(out)))
     return out

@@ -81,12 +81,14 @@ def test_cr1_monotropic_criterion():
     assert np.abs(E[:4, 4:]).max() < 1e-2 * np.abs(E).max()
 
 
-def test_gradient_matches_finite_differences():
-    shape = (5, 5, 5)
+@pytest.mark.parametrize("nen,shape", [(8, (5, 5, 5)), (20, (4, 4, 4))])
+def test_gradient_matches_finite_differences(nen, shape):
+    """dL/dE_e of the oracle (HEX08 or HEX20 micro elements, HEX08 macro) against
+    central differences of full oracle solves at relative step 1e-4."""
     phi = random_mask(shape, 0.65, 21)
     phi[0] = 1
     delta, xa = 0.01, (2, 2, 2)
-    nodes, elems = O.mesh(phi, 1, 8, (delta,) * 3, xa)
+    nodes, elems = O.mesh(phi, 1, nen, (delta,) * 3, xa)
     mn, mx, dim = domain_box(shape[::-1], delta, xa)
     a = shape[2] * delta * delta
     rng = np.random.default_rng(4)
