@@ -165,8 +165,7 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res);
 /* sensitivity of L = sum_ij dL_dC[ij] C_ij to every voxel modulus of domain i
  * after dtc_batch_run (self-adjoint: uses the stored load-case solutions,
  * no extra solve).  dL_dE: n^3 per voxel (0 where no element).  HEX08 or HEX20
- * micro elements under the HEX08 macro element (DTC_ERR_UNSUPPORTED for the
- * HEX20 macro element). */
+ * micro elements, HEX08 or HEX20 macro element. */
 int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_dE);
 /* timing of the last run: average duration (ms) of the CG matvec kernel
  * launches measured with HIP events on the batch's stream, the number of

@@ -760,7 +760,6 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
     if (!b->ran) return set_error(DTC_ERR_BAD_ARGUMENT, "dtc_batch_grad before dtc_batch_run");
     if (i < 0 || i >= (int64_t)b->d.size()) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %lld", (long long)i);
     const dtc_domain_desc& d = b->d[i];
-    if (d.macro_order != 1) return set_error(DTC_ERR_UNSUPPORTED, "the sensitivity implements the HEX08 macro element");
     const int gi = b->g0[i];
     SolverBatch& sb = b->subs[b->sbi[i]]->sb;
     const size_t nv = (size_t)d.n[0] * d.n[1] * d.n[2];
@@ -768,25 +767,30 @@ int dtc_batch_grad(dtc_batch* b, int64_t i, const double dL_dC[36], double* dL_d
         std::memset(dL_dE, 0, nv * 8);
         return DTC_OK;
     }
-    // M(b,j) = meps_j sum_i G_ij sum_a W_ia vv_eps^-1(b,a); vv at eps = 1e-6, the
-    // value the boundary tables use (f-src/mod_write_deck.f90:68)
-    double vv[576], W[144], M[144];
-    vv_lin(1.e-6, vv);
-    DTC_TRY(invert(vv, R));
+    // M(b,j) = meps_j sum_i G_ij sum_a W_ia vv_eps^-1(b,a) over the RM macro load
+    // cases (24 or 60, the vv and fit weights of dtc's effective()); vv at eps =
+    // 1e-6, the value the boundary tables use (f-src/mod_write_deck.f90:68)
+    const int RM = macro_lcs(d.macro_order), ng = b->ng[i];
+    std::vector<double> vv((size_t)RM * RM), W(6 * 60), M((size_t)ng * R * 6, 0.0);
+    if (d.macro_order == 1) vv_lin(1.e-6, vv.data());
+    else vv_quad(1.e-6, vv.data());
+    DTC_TRY(invert(vv.data(), RM));
     double xdp[3] = {(double)d.n[0] * d.delta[0], (double)d.n[1] * d.delta[1], (double)d.n[2] * d.delta[2]};
-    fv_weights(xdp, W);
-    for (int bb = 0; bb < R; ++bb)
+    if (d.macro_order == 1) fv_weights(xdp, W.data());
+    else fv_weights_quad(xdp, W.data());
+    if (ng * R < RM) return set_error(DTC_ERR_DEVICE, "%d load-case groups for %d load cases", ng, RM);
+    for (int bb = 0; bb < RM; ++bb)
         for (int j = 0; j < 6; ++j) {
             double s = 0.0;
             for (int ii = 0; ii < 6; ++ii) {
                 double t = 0.0;
-                for (int a = 0; a < R; ++a) t += W[ii + 6 * a] * vv[bb + R * a];
+                for (int a = 0; a < RM; ++a) t += W[ii + 6 * a] * vv[bb + RM * a];
                 s += dL_dC[ii + 6 * j] * t;
             }
             M[bb * 6 + j] = s * MEPS[j];
         }
     std::vector<double> g(nv);
-    DTC_TRY(batch_grad(sb, gi, M, g.data()));
+    DTC_TRY(batch_grad(sb, gi, ng, M.data(), g.data()));
     const double sc = b->escale[i];   // d(emod)/d(E_v)
     for (size_t v = 0; v < nv; ++v) dL_dE[v] = g[v] * sc;
     return DTC_OK;

@@ -1671,10 +1671,14 @@ __global__ __launch_bounds__(NT_UPD) void k_fields(const BrickInfo* __restrict__
 }
 
 // ---------------------------------------------------------------- sensitivity
-// h[node][j][d] = sum_lc M[lc][j] u_lc(node, d), u = x + x_bc
+// h[node][j][d] (+)= sum_lc M[lc][j] u_lc(node, d), u = x + x_bc, over the load cases
+// of domain group dom (M: its 24 rows); h is indexed by the node numbers of group
+// h_begin's range (the groups of one domain share one mesh and its numbering),
+// add = 0 for the first group
 __global__ __launch_bounds__(NT_UPD) void k_grad_h(const BrickInfo* __restrict__ bricks, const DomInfo* __restrict__ doms,
                                                    const int* __restrict__ lat2node, const double* __restrict__ x,
-                                                   const double* __restrict__ Mj, int dom, double* __restrict__ h) {
+                                                   const double* __restrict__ Mj, int dom, int64_t h_begin, int add,
+                                                   double* __restrict__ h) {
     const BrickInfo B = bricks[blockIdx.x];
     if (B.dom != dom) return;
     const DomInfo D = doms[B.dom];
@@ -1686,18 +1690,19 @@ __global__ __launch_bounds__(NT_UPD) void k_grad_h(const BrickInfo* __restrict__
         int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
         if (q < 0) continue;
         bool bc = is_bc(D, gx, gy, gz);
-        double N[8];
+        double N[20];
         if (bc) {
             double c[3];
             node_coord(D, gx, gy, gz, c);
-            macro_shape8(c, D.min_c, D.max_c, N);
+            macro_shape(D.macro_order, c, D.min_c, D.max_c, N);
         }
         double s = 0.0;
         for (int lc = 0; lc < R; ++lc) {
-            double u = bc ? bc_value(N, d, lc) : x[((int64_t)q * 3 + d) * R + lc];
+            double u = bc ? bc_value_m(D.macro_order, N, d, D.lc_base + lc) : x[((int64_t)q * 3 + d) * R + lc];
             s += Mj[lc * 6 + j] * u;
         }
-        h[((int64_t)q * 6 + j) * 3 + d] = s;
+        double* hp = h + ((q - D.node_begin + h_begin) * 6 + j) * 3 + d;
+        *hp = add ? *hp + s : s;
     }
 }
 
@@ -1707,10 +1712,10 @@ __device__ inline double u_at(const DomInfo& D, const int* lat2node, const doubl
     int q = lat2node[D.lat_off + gx + (int64_t)Lx * (gy + (int64_t)Ly * gz)];
     if (q < 0) return 0.0;
     if (is_bc(D, gx, gy, gz)) {
-        double c[3], N[8];
+        double c[3], N[20];
         node_coord(D, gx, gy, gz, c);
-        macro_shape8(c, D.min_c, D.max_c, N);
-        return bc_value(N, d, lc);
+        macro_shape(D.macro_order, c, D.min_c, D.max_c, N);
+        return bc_value_m(D.macro_order, N, d, D.lc_base + lc);
     }
     return x[((int64_t)q * 3 + d) * R + lc];
 }
@@ -3755,21 +3760,23 @@ int batch_fields(SolverBatch& b, int i, int ncol, double* u_out, double* r_out) 
     return DTC_OK;
 }
 
-int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host) {
-    if (i < 0 || i >= b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %d", i);
-    if (b.micro20) return grad20(b, i, M, dL_dE_host);
+int batch_grad(SolverBatch& b, int i, int ng, const double* M, double* dL_dE_host) {
+    if (i < 0 || ng < 1 || i + ng > b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain groups %d..%d", i, i + ng - 1);
+    if (b.micro20) return grad20(b, i, ng, M, dL_dE_host);
     const DomInfo& D = b.h_dom[i];
     const int64_t nvox = (int64_t)D.nx * D.ny * D.nz;
     const int64_t nn = b.n_nodes > 0 ? b.n_nodes : 1;
     DevMem h, Md, g;
     DTC_TRY(h.alloc((size_t)nn * 18 * 8));
-    DTC_TRY(Md.alloc(24 * 6 * 8));
+    DTC_TRY(Md.alloc((size_t)ng * R * 6 * 8));
     DTC_TRY(g.alloc((size_t)nvox * 8));
-    DTC_HIP(hipMemcpyAsync(Md.p, M, 24 * 6 * 8, hipMemcpyHostToDevice, b.stream));
-    if (D.brick_end > D.brick_begin) {
-        k_grad_h<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.lat2node.as<int>(),
-                                                      b.x64.as<double>(), Md.as<double>(), i, h.as<double>());
-    }
+    DTC_HIP(hipMemcpyAsync(Md.p, M, (size_t)ng * R * 6 * 8, hipMemcpyHostToDevice, b.stream));
+    for (int k = 0; k < ng; ++k)
+        if (b.h_dom[i + k].brick_end > b.h_dom[i + k].brick_begin)
+            k_grad_h<<<b.n_bricks, NT_UPD, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(),
+                                                          b.lat2node.as<int>(), b.x64.as<double>(),
+                                                          Md.as<double>() + (size_t)k * R * 6, i + k, D.node_begin,
+                                                          k > 0, h.as<double>());
     k_grad_elem<<<nb(nvox, 128), 128, 0, b.stream>>>(b.dom.as<DomInfo>(), b.lat2node.as<int>(), b.emod.as<double>(),
                                                      b.x64.as<double>(), h.as<double>(), cast_wk<double>(b.wk64), i,
                                                      g.as<double>());

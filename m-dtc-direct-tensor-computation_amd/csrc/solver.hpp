@@ -225,9 +225,11 @@ int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout
 int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done, double* q64 = nullptr);
 int project20(SolverBatch& b);
 int fields20(SolverBatch& b, int i, int ncol, double* u_dev, double* r_dev);
-// dL/dE per voxel of domain group i of a HEX20 batch (HEX08 macro: M is 24 x 6)
-int grad20(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host);
-// sensitivity dL/dE per voxel of domain i.
-int batch_grad(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host);
+// dL/dE per voxel of the domain whose load cases are groups i..i+ng-1 of a HEX20
+// batch (M: 24 ng x 6, rows of absent load cases zero)
+int grad20(SolverBatch& b, int i, int ng, const double* M, double* dL_dE_host);
+// sensitivity dL/dE per voxel of the domain whose load cases are groups i..i+ng-1
+// (M: 24 ng x 6, rows of absent load cases zero)
+int batch_grad(SolverBatch& b, int i, int ng, const double* M, double* dL_dE_host);
 
 }  // namespace dtc

@@ -702,9 +702,11 @@ int project20(SolverBatch& b) {
 }
 
 // ---------------------------------------------------------------- sensitivity (HEX20 micro, HEX08 macro)
-// h[q][j][d] = sum_lc M[lc][j] u_lc(q, d), u = x + x_bc (the HEX08 form in k_grad_h)
+// h[q][j][d] (+)= sum_lc M[lc][j] u_lc(q, d), u = x + x_bc, over the 24 load cases of
+// one domain group (the HEX08 form in k_grad_h); q is the node's local number, the
+// same in every group of a domain; add = 0 for the first group
 __global__ __launch_bounds__(256) void k20_grad_h(Dom20Dev G, DomInfo D, const double* __restrict__ x,
-                                                  const double* __restrict__ Mj, double* __restrict__ h) {
+                                                  const double* __restrict__ Mj, int add, double* __restrict__ h) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)G.n_nodes * 18) return;
     const int q = (int)(i / 18), j = (int)(i % 18) / 3, d = (int)(i % 3);
@@ -714,7 +716,7 @@ __global__ __launch_bounds__(256) void k20_grad_h(Dom20Dev G, DomInfo D, const d
         const double u = bc ? bc20(D, G, q, d, lc) : x[((G.node_begin + q) * 3 + d) * R + lc];
         s += Mj[lc * 6 + j] * u;
     }
-    h[i] = s;
+    h[i] = add ? h[i] + s : s;
 }
 
 // dL/dE_v = sum_j h_j,e^T Khat u_j,e per element (one wave per element: lane r < 60
@@ -759,22 +761,27 @@ __global__ __launch_bounds__(64 * G20_EPB) void k20_grad_elem(Dom20Dev G, DomInf
     if (lane == 0) g[evox[e]] = part;
 }
 
-int grad20(SolverBatch& b, int i, const double M[24 * 6], double* dL_dE_host) {
-    if (i < 0 || i >= b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain index %d", i);
+int grad20(SolverBatch& b, int i, int ng, const double* M, double* dL_dE_host) {
+    if (i < 0 || ng < 1 || i + ng > b.n_dom) return set_error(DTC_ERR_BAD_ARGUMENT, "domain groups %d..%d", i, i + ng - 1);
     const Dom20& G = b.d20[i];
     const Dom20Dev& Gd = b.h_d20dev[i];
     const DomInfo& D = b.h_dom[i];
     hipStream_t s = b.stream;
     const int64_t nvox = G.nvox, ne = b.h_elems[i];
     DevMem h, Md, g;
-    DTC_TRY(Md.alloc(24 * 6 * 8));
+    DTC_TRY(Md.alloc((size_t)ng * R * 6 * 8));
     DTC_TRY(g.alloc((size_t)std::max<int64_t>(nvox, 1) * 8));
-    DTC_HIP(hipMemcpyAsync(Md.p, M, 24 * 6 * 8, hipMemcpyHostToDevice, s));
+    DTC_HIP(hipMemcpyAsync(Md.p, M, (size_t)ng * R * 6 * 8, hipMemcpyHostToDevice, s));
     DTC_HIP(hipMemsetAsync(g.p, 0, (size_t)std::max<int64_t>(nvox, 1) * 8, s));   // voxels without an element: 0
     if (ne > 0 && Gd.n_nodes > 0) {
         DTC_TRY(h.alloc((size_t)Gd.n_nodes * 18 * 8));
-        k20_grad_h<<<nb((int64_t)Gd.n_nodes * 18, 256), 256, 0, s>>>(Gd, D, b.x64.as<double>(), Md.as<double>(),
-                                                                     h.as<double>());
+        for (int k = 0; k < ng; ++k) {
+            const Dom20Dev& Gk = b.h_d20dev[i + k];
+            if (Gk.n_nodes != Gd.n_nodes) return set_error(DTC_ERR_DEVICE, "load-case groups of one domain differ");
+            k20_grad_h<<<nb((int64_t)Gd.n_nodes * 18, 256), 256, 0, s>>>(Gk, b.h_dom[i + k], b.x64.as<double>(),
+                                                                         Md.as<double>() + (size_t)k * R * 6, k > 0,
+                                                                         h.as<double>());
+        }
         k20_grad_elem<<<nb(ne, G20_EPB), 64 * G20_EPB, 0, s>>>(Gd, D, b.k20_64.as<double>(), b.x64.as<double>(),
                                                                h.as<double>(), G.evox.as<int>(), ne, g.as<double>());
     }

@@ -1040,19 +1040,24 @@ int dtco_rotation_search(const double EE[36], int criterion, double step_deg, do
  * value init_displ_hexe8 uses).  Hence dC_ij/dE_e = meps_j sum_a W_ia
  * w_a,e^T Khat u_j,e and dL/dE_e = sum_j h_j,e^T Khat u_j,e, for HEX08 and HEX20
  * micro elements (nen nodes, Khat of the element type) under the HEX08 macro
- * element's 24 load cases. */
+ * element's 24 load cases or the HEX20 macro element's 60 (its consistent vv,
+ * as in dtco_effective). */
 int dtco_grad_modulus(const double* nodes, int64_t n_nodes, const int64_t* elems,
-                      int64_t n_elems, int nen, double nu, double a, const double min_c[3],
-                      const double max_c[3], const double x_D_phy[3], double rve_strain,
-                      const double* u, const double G[36], double* dL_dE) {
+                      int64_t n_elems, int nen, int macro_order, double nu, double a,
+                      const double min_c[3], const double max_c[3], const double x_D_phy[3],
+                      double rve_strain, const double* u, const double G[36], double* dL_dE) {
     (void)nodes; (void)min_c; (void)max_c; (void)rve_strain;
     if (nen != 8 && nen != 20) return 5;
-    const int R = 24, nd = 3 * nen;
+    if (macro_order != 1 && macro_order != 2) return 5;
+    const int R = macro_order == 1 ? 24 : 60, nd = 3 * nen;
     const int64_t m = 3 * n_nodes;
-    double vv[576], W[6 * 24], M[24 * 6], Khat[3600];
-    dtco_vv_lin(1.e-6, vv);
+    double vv[3600];
+    double W[6 * 60], M[60 * 6], Khat[3600];
+    if (macro_order == 1) dtco_vv_lin(1.e-6, vv);
+    else dtco_vv_quad_consistent(1.e-6, vv);
     if (invert(vv, R)) return 4;
-    fv_weights_lin(x_D_phy, W);
+    if (macro_order == 1) fv_weights_lin(x_D_phy, W);
+    else fv_weights_quad(x_D_phy, W);
     /* M(b,j) = meps_j sum_i G_ij sum_a W_ia vvinv(b,a) */
     for (int b = 0; b < R; ++b)
         for (int j = 0; j < 6; ++j) {
@@ -1067,7 +1072,7 @@ int dtco_grad_modulus(const double* nodes, int64_t n_nodes, const int64_t* elems
     dtco_ke(nen, 1.0, nu, a, Khat);
     for (int64_t e = 0; e < n_elems; ++e) {
         const int64_t* en = elems + (int64_t)nen * e;
-        double ue[24][60];   /* [lc][local dof] */
+        double ue[60][60];   /* [lc][local dof] */
         for (int lc = 0; lc < R; ++lc)
             for (int l = 0; l < nen; ++l)
                 for (int d = 0; d < 3; ++d)

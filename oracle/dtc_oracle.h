@@ -98,12 +98,13 @@ int dtco_rotation_search(const double EE[36], int criterion, double step_deg, do
 int dtco_rotate66(const double EE[36], const double n[3], double alpha, double T[36]);
 
 /* Sensitivity of the 6x6 effective stiffness: given the 24 load-case
- * displacements u (3N_n x 24), returns dL/dE_e for L = sum G_ij C_ij with
- * G = dL/dC (6x6 column-major).  HEX08 or HEX20 micro (nen) + HEX08 macro. */
+ * displacements u (3N_n x 24, or x 60 for the HEX20 macro element), returns
+ * dL/dE_e for L = sum G_ij C_ij with G = dL/dC (6x6 column-major).  HEX08 or
+ * HEX20 micro (nen), HEX08 or HEX20 macro (macro_order). */
 int dtco_grad_modulus(const double* nodes, int64_t n_nodes, const int64_t* elems,
-                      int64_t n_elems, int nen, double nu, double a, const double min_c[3],
-                      const double max_c[3], const double x_D_phy[3], double rve_strain,
-                      const double* u, const double G[36], double* dL_dE);
+                      int64_t n_elems, int nen, int macro_order, double nu, double a,
+                      const double min_c[3], const double max_c[3], const double x_D_phy[3],
+                      double rve_strain, const double* u, const double G[36], double* dL_dE);
 
 #ifdef __cplusplus
 }

@@ -204,15 +204,15 @@ def rotate66(EE, n, alpha):
     return T.reshape(6, 6).T.copy()
 
 
-def grad_modulus(nodes, elems, nu, a, min_c, max_c, x_D_phy, u, G, rve_strain=1e-6):
+def grad_modulus(nodes, elems, nu, a, min_c, max_c, x_D_phy, u, G, rve_strain=1e-6, macro_order=1):
     nodes = np.ascontiguousarray(nodes, np.float64)
     elems = np.ascontiguousarray(elems, np.int64)
-    uc = np.ascontiguousarray(u, np.float64)          # (24, 3N)
+    uc = np.ascontiguousarray(u, np.float64)          # (24 or 60, 3N)
     Gc = np.ascontiguousarray(np.asarray(G, np.float64).T)
     out = np.zeros(elems.shape[0])
     _chk("grad_modulus", lib().dtco_grad_modulus(
         _p(nodes), C.c_int64(nodes.shape[0]), _p(elems), C.c_int64(elems.shape[0]), C.c_int(elems.shape[1]),
-        C.c_double(nu),
+        C.c_int(macro_order), C.c_double(nu),
         C.c_double(a), _p(_f64(*min_c)), _p(_f64(*max_c)), _p(_f64(*x_D_phy)), C.c_double(rve_strain),
         _p(uc), _p(Gc), _p(out)))
     return out

@@ -102,21 +102,19 @@ def test_hex20_micro_batch_with_failures_and_per_voxel_modulus():
         assert rb[3]["rot"][c]["alpha"] == pytest.approx(s4["rot"][c]["alpha"], abs=1e-6)
 
 
-def test_hex20_micro_rejects_mixed_element_types_and_hex20_macro_gradient():
+def test_hex20_micro_rejects_mixed_element_types():
     dtc = _dtc()
     phi = np.ones((4, 4, 4), np.int8)
     with pytest.raises(Exception):
         dtc.solve_batch([dtc.Domain(phi), dtc.Domain(phi, micro_nen=20)], precision=1)
-    with pytest.raises(Exception):
-        with dtc.Batch([dtc.Domain(phi, micro_nen=20, macro_order=2)], precision=1) as b:
-            b.run()
-            b.grad(0, np.eye(6))
 
 
-def test_hex20_gradient_matches_oracle_and_finite_differences():
-    """dL/dE per voxel of a HEX20 micro domain (HEX08 macro) from the stored solutions
-    against the oracle's HEX20 sensitivity (1e-6 of max |g|, zero on void voxels)
-    and against central differences of full GPU solves (relative step 1e-4)."""
+@pytest.mark.parametrize("nen,macro_order", [(20, 1), (8, 2), (20, 2)])
+def test_hex20_gradient_matches_oracle_and_finite_differences(nen, macro_order):
+    """dL/dE per voxel with HEX20 micro and/or HEX20 macro elements (the 60 load
+    cases as 3 groups) from the stored solutions against the oracle's sensitivity
+    (1e-6 of max |g|, zero on void voxels) and against central differences of full
+    GPU solves (relative step 1e-4)."""
     dtc, O = _dtc(), _O()
     from helpers import random_mask
     rng = np.random.default_rng(19)
@@ -124,17 +122,18 @@ def test_hex20_gradient_matches_oracle_and_finite_differences():
     phi[0] = 1
     modulus = rng.uniform(2000, 8000, phi.shape)
     G = rng.standard_normal((6, 6))
-    d = dtc.Domain(phi, delta=(0.01,) * 3, modulus=modulus, micro_nen=20)
+    mk = lambda m: dtc.Domain(phi, delta=(0.01,) * 3, modulus=m, micro_nen=nen, macro_order=macro_order)
+    d = mk(modulus)
     with dtc.Batch([d], precision=0, rel_tol=1e-11, rotation=False) as b:
         r = b.run()[0]
         g = b.grad(0, G)
     assert r["status"] == 0
-    nodes, elems = O.mesh(d.phi, d.llimit, 20, d.delta, d.xa_n)
+    nodes, elems = O.mesh(d.phi, d.llimit, nen, d.delta, d.xa_n)
     ijk = np.rint(nodes[elems[:, 0] - 1] / 0.01 - 1).astype(int)
     mn, mx, dim = d.box()
     o = O.solve_domain(nodes, elems, d.E, d.nu, d.a(), mn, mx, dim, tol=1e-13, want_u=True,
-                       emod=modulus[ijk[:, 2], ijk[:, 1], ijk[:, 0]])
-    go = O.grad_modulus(nodes, elems, d.nu, d.a(), mn, mx, dim, o["u"], G)
+                       emod=modulus[ijk[:, 2], ijk[:, 1], ijk[:, 0]], macro_order=macro_order)
+    go = O.grad_modulus(nodes, elems, d.nu, d.a(), mn, mx, dim, o["u"], G, macro_order=macro_order)
     gv = g[ijk[:, 2], ijk[:, 1], ijk[:, 0]]
     assert _rel(gv, go) < 1e-6
     assert np.all(g[phi == 0] == 0)
@@ -144,8 +143,7 @@ def test_hex20_gradient_matches_oracle_and_finite_differences():
         for sgn in (1, -1):
             m2 = modulus.copy().ravel()
             m2[k] += sgn * h
-            x = dtc.solve_domain(dtc.Domain(phi, delta=(0.01,) * 3, modulus=m2.reshape(phi.shape), micro_nen=20),
-                                 precision=0, rel_tol=1e-11, rotation=False)
+            x = dtc.solve_domain(mk(m2.reshape(phi.shape)), precision=0, rel_tol=1e-11, rotation=False)
             vals.append(np.sum(G * x["C"]))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - g.ravel()[k]) < 1e-4 * np.abs(g).max()

@@ -81,10 +81,10 @@ def test_cr1_monotropic_criterion():
     assert np.abs(E[:4, 4:]).max() < 1e-2 * np.abs(E).max()
 
 
-@pytest.mark.parametrize("nen,shape", [(8, (5, 5, 5)), (20, (4, 4, 4))])
-def test_gradient_matches_finite_differences(nen, shape):
-    """dL/dE_e of the oracle (HEX08 or HEX20 micro elements, HEX08 macro) against
-    central differences of full oracle solves at relative step 1e-4."""
+@pytest.mark.parametrize("nen,shape,mo", [(8, (5, 5, 5), 1), (20, (4, 4, 4), 1), (8, (4, 4, 4), 2), (20, (3, 3, 4), 2)])
+def test_gradient_matches_finite_differences(nen, shape, mo):
+    """dL/dE_e of the oracle (HEX08 or HEX20 micro elements, HEX08 or HEX20 macro
+    element) against central differences of full oracle solves at relative step 1e-4."""
     phi = random_mask(shape, 0.65, 21)
     phi[0] = 1
     delta, xa = 0.01, (2, 2, 2)
@@ -96,12 +96,12 @@ def test_gradient_matches_finite_differences(nen, shape):
     G = rng.standard_normal((6, 6))
 
     def loss(em):
-        r = O.solve_domain(nodes, elems, 1.0, 0.3, a, mn, mx, dim, tol=1e-13, emod=em)
-        return np.sum(G * O.effective(r["ff"], dim)[1]), r
+        r = O.solve_domain(nodes, elems, 1.0, 0.3, a, mn, mx, dim, tol=1e-13, emod=em, macro_order=mo)
+        return np.sum(G * O.effective(r["ff"], dim, macro_order=mo)[1]), r
 
     L0, r0 = loss(emod)
-    r0 = O.solve_domain(nodes, elems, 1.0, 0.3, a, mn, mx, dim, tol=1e-13, emod=emod, want_u=True)
-    g = O.grad_modulus(nodes, elems, 0.3, a, mn, mx, dim, r0["u"], G)
+    r0 = O.solve_domain(nodes, elems, 1.0, 0.3, a, mn, mx, dim, tol=1e-13, emod=emod, want_u=True, macro_order=mo)
+    g = O.grad_modulus(nodes, elems, 0.3, a, mn, mx, dim, r0["u"], G, macro_order=mo)
     for e in rng.choice(elems.shape[0], 6, replace=False):
         h = 1e-4 * emod[e]
         ep, em = emod.copy(), emod.copy()
