@@ -999,7 +999,7 @@ constexpr double ORTH_DROP = 1e-10;   // of P^T P
 // four waves take every fourth vector row.  The loads of the next GU rows of a
 // wave are issued before the products of the current ones (a slice is a long
 // sequential walk per wave); the waves' sums are added in wave order.
-template <int NGR, typename TA = float, typename TB = float>
+template <int NGR, typename TA = float, typename TB = float, int GU = 4>
 __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices, const int* __restrict__ done,
                                                const TA* __restrict__ A, const TB* __restrict__ B,
                                                const TB* __restrict__ C, double* __restrict__ gpart) {
@@ -1013,7 +1013,6 @@ __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices,
     for (int t = 0; t < NGR; ++t)
 #pragma unroll
         for (int k = 0; k < 9; ++k) g[t][k] = 0.0;
-    constexpr int GU = 4;
     const int64_t r1 = (int64_t)sl.z * 3;
     // two register sets: the next GU rows load into one while the products of the
     // other run; full steps (all GU rows in the slice) load without guards
@@ -1073,6 +1072,107 @@ __global__ __launch_bounds__(GR_NT) void k_gram(const int4* __restrict__ slices,
 #pragma unroll
         for (int t = 0; t < NGR; ++t) product(a, b[t], g[t]);
     }
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w)
+#pragma unroll
+            for (int t = 0; t < NGR; ++t)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const int e = t * RR + (ci + i) * R + cj + j;
+                        s_g[e] = w == 0 ? g[t][3 * i + j] : s_g[e] + g[t][3 * i + j];
+                    }
+        __syncthreads();
+    }
+    double* out = gpart + (int64_t)blockIdx.x * 2 * RR;
+    for (int e = tid; e < 2 * RR; e += GR_NT) out[e] = e < NGR * RR ? s_g[e] : 0.0;
+}
+
+// The same Gram partials with the rows staged through LDS: the workgroup loads
+// blocks of GB_ROWS vector rows of A, B (and C) with coalesced 16-byte loads (the
+// next block's loads issued before the current block's products), and lane (i, j)
+// reads its 3 + 3 (+ 3) values of a row from LDS, where the 8 lanes sharing an
+// address are served by one broadcast.  (Loading them from global memory sends
+// each value through the vector memory path 8 times.)  Wave w takes rows w, w + 4,
+// ... of each block in order, so every wave sums the same rows in the same order
+// as k_gram and the results are identical.
+constexpr int GB_ROWS = 32;
+template <int NGR, typename TA = float, typename TB = float>
+__global__ __launch_bounds__(GR_NT) void k_gram_lds(const int4* __restrict__ slices, const int* __restrict__ done,
+                                                   const TA* __restrict__ A, const TB* __restrict__ B,
+                                                   const TB* __restrict__ C, double* __restrict__ gpart) {
+    constexpr int EA = 16 / sizeof(TA), EB = 16 / sizeof(TB);      // elements per 16-byte vector
+    constexpr int VA = GB_ROWS * R / EA, VB = GB_ROWS * R / EB;     // 16-byte vectors per block and array
+    constexpr int VT = VA + NGR * VB;
+    constexpr int NPF = (VT + GR_NT - 1) / GR_NT;                   // prefetch vectors per thread
+    __shared__ uint4 sAv[VA];
+    __shared__ uint4 sBv[NGR][VB];
+    __shared__ double s_g[NGR * RR];
+    const int4 sl = slices[blockIdx.x];   // domain, first node, end node
+    if (done[sl.x]) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ci = 3 * (lane >> 3), cj = 3 * (lane & 7);
+    const int64_t r0 = (int64_t)sl.y * 3, r1 = (int64_t)sl.z * 3;
+    double g[NGR][9];
+#pragma unroll
+    for (int t = 0; t < NGR; ++t)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) g[t][k] = 0.0;
+    uint4 pf[NPF];
+    // vector v of block row0: array (0 = A, 1 = B, 2 = C), rows beyond r1 read as zero
+    auto load = [&](int64_t row0) {
+#pragma unroll
+        for (int k = 0; k < NPF; ++k) {
+            const int v = tid + k * GR_NT;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (v < VA) {
+                const int64_t e = row0 * R + (int64_t)v * EA;
+                if (e < r1 * R) val = reinterpret_cast<const uint4*>(A + e)[0];
+            } else if (v < VT) {
+                const int w = v - VA, t = w / VB, u = w % VB;
+                const int64_t e = row0 * R + (int64_t)u * EB;
+                const TB* src = t == 0 ? B : C;
+                if (e < r1 * R) val = reinterpret_cast<const uint4*>(src + e)[0];
+            }
+            pf[k] = val;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int k = 0; k < NPF; ++k) {
+            const int v = tid + k * GR_NT;
+            if (v < VA) sAv[v] = pf[k];
+            else if (v < VT) sBv[(v - VA) / VB][(v - VA) % VB] = pf[k];
+        }
+    };
+    const TA* sA = reinterpret_cast<const TA*>(sAv);
+    if (r0 < r1) load(r0);
+    for (int64_t row0 = r0; row0 < r1; row0 += GB_ROWS) {
+        __syncthreads();                              // the previous block's products are done
+        store();
+        __syncthreads();
+        if (row0 + GB_ROWS < r1) load(row0 + GB_ROWS);
+#pragma unroll 2
+        for (int k = 0; k < GB_ROWS / 4; ++k) {
+            const int rr = wave + 4 * k;
+            if (row0 + rr >= r1) break;
+            double ad[3], bd[NGR][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                ad[i] = (double)sA[rr * R + ci + i];
+#pragma unroll
+                for (int t = 0; t < NGR; ++t) bd[t][i] = (double)reinterpret_cast<const TB*>(sBv[t])[rr * R + cj + i];
+            }
+#pragma unroll
+            for (int t = 0; t < NGR; ++t)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) g[t][3 * i + j] = fma(ad[i], bd[t][j], g[t][3 * i + j]);
+        }
+    }
+    __syncthreads();
     for (int w = 0; w < 4; ++w) {
         if (wave == w)
 #pragma unroll
@@ -3135,15 +3235,27 @@ int block_setup(SolverBatch& b) {
 
 // per-domain Grams a^T b and a^T c (c may be null) into gsum
 template <typename T> struct NoDeduce { using type = T; };
+// Gram form: rows staged through LDS (default) or read by every lane from global
+// memory (DTC_GRAM_LDS=0; the same sums, for measurements)
+inline bool gram_lds() {
+    const char* e = std::getenv("DTC_GRAM_LDS");
+    return !(e && e[0] == '0');
+}
 template <typename TA = float, typename TB = float>
 int block_gram(SolverBatch& b, const TA* a, const TB* bb, const typename NoDeduce<TB>::type* c) {
     hipStream_t s = b.stream;
     int* done = b.done.as<int>();
-    if (b.n_slices > 0 && c)
-        k_gram<2, TA, TB><<<b.n_slices, GR_NT, 0, s>>>(b.slices.as<int4>(), done, a, bb, c, b.gpart.as<double>());
-    else if (b.n_slices > 0)
-        k_gram<1, TA, TB><<<b.n_slices, GR_NT, 0, s>>>(b.slices.as<int4>(), done, a, bb, nullptr,
-                                                       b.gpart.as<double>());
+    if (b.n_slices > 0) {
+        const int4* sl = b.slices.as<int4>();
+        double* gp = b.gpart.as<double>();
+        if (gram_lds()) {
+            if (c) k_gram_lds<2, TA, TB><<<b.n_slices, GR_NT, 0, s>>>(sl, done, a, bb, c, gp);
+            else k_gram_lds<1, TA, TB><<<b.n_slices, GR_NT, 0, s>>>(sl, done, a, bb, nullptr, gp);
+        } else {
+            if (c) k_gram<2, TA, TB><<<b.n_slices, GR_NT, 0, s>>>(sl, done, a, bb, c, gp);
+            else k_gram<1, TA, TB><<<b.n_slices, GR_NT, 0, s>>>(sl, done, a, bb, nullptr, gp);
+        }
+    }
     k_gram_sum<<<dim3(b.n_dom, 2 * RR / 64), 256, 0, s>>>(b.dslice.as<int>(), done, b.gpart.as<double>(),
                                                          b.gsum.as<double>());
     DTC_HIP(hipGetLastError());

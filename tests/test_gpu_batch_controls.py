@@ -220,3 +220,21 @@ def test_whole_segment_output_stores_equal_per_chunk_stores(monkeypatch):
             assert a["iterations"] == c["iterations"]
             for k in ("C", "K_macro", "ff"):
                 assert np.array_equal(a[k], c[k]), (pc, k)
+
+
+def test_lds_staged_gram_sums_equal_the_global_form(monkeypatch):
+    """The block PCG's Gram sums staged through LDS add the same rows in the same
+    order per wave as the form that reads every operand from global memory
+    (DTC_GRAM_LDS=0): the solves agree bit for bit, on FP32 and FP64-vector blocks."""
+    dtc = _dtc()
+    for doms in ([_binary(dtc, 24, 81), _binary(dtc, 20, 82)], [_grey(dtc, 20, 83)]):
+        kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+        monkeypatch.setenv("DTC_GRAM_LDS", "0")
+        ra = dtc.solve_batch(doms, **kw)
+        monkeypatch.delenv("DTC_GRAM_LDS")
+        rb = dtc.solve_batch(doms, **kw)
+        for a, c in zip(ra, rb):
+            assert a["status"] == c["status"] == 0
+            assert a["iterations"] == c["iterations"]
+            for k in ("C", "K_macro", "ff"):
+                assert np.array_equal(a[k], c[k]), k

@@ -408,16 +408,20 @@ def test_mixed_macro_orders_in_one_batch():
     phi = synth.trabecular(11, seed=72, sigma=1.5, bvtv=0.45)
     d1 = dtc.Domain(phi, delta=(0.01,) * 3)
     d2 = dtc.Domain(phi, delta=(0.01,) * 3, macro_order=2)
-    rb = dtc.solve_batch([d1, d2, d1], precision=1, rel_tol=1e-8, rotation=True, precond=1)
+    G = np.random.default_rng(3).standard_normal((6, 6))
+    with dtc.Batch([d1, d2, d1], precision=1, rel_tol=1e-8, rotation=True, precond=1) as bb:
+        rb = bb.run()
+        gb = [bb.grad(k, G) for k in range(3)]
     assert [x["status"] for x in rb] == [0, 0, 0]
     assert rb[0]["ff"].shape == (24, 24) and rb[1]["ff"].shape == (60, 60)
     # load cases 1-6 are the same affine fields: identical reactions, C within the fit's face weights
     s1 = dtc.solve_domain(d1, precision=1, rel_tol=1e-8, rotation=False)
     assert _rel(rb[0]["C"], s1["C"]) < 1e-6 and _rel(rb[2]["C"], s1["C"]) < 1e-6
-    with pytest.raises(Exception):
-        with dtc.Batch([d2], precision=1) as b:
+    # the sensitivity of each domain in the mixed batch equals that of the domain alone
+    for k, d in ((0, d1), (1, d2)):
+        with dtc.Batch([d], precision=1, rel_tol=1e-8, rotation=False, precond=1) as b:
             b.run()
-            b.grad(0, np.eye(6))
+            assert _rel(gb[k], b.grad(0, G)) < 1e-6, k
 
 
 # ------------------------------------------------------------------ sub-batches
