@@ -1299,24 +1299,24 @@ __global__ __launch_bounds__(BS_NT) void k_bsolve(const int* __restrict__ done, 
     }
 }
 
-// X += P alpha, R -= Q alpha on the rows of each brick (alpha: k_bsolve mode 0),
-// and per-brick sums of r.r per load case.  Of the 6 threads of a row, thread q
+// X += P alpha, R -= Q alpha on the rows of each node slice of a domain (the Gram
+// slices; alpha: k_bsolve mode 0), and per-slice sums of r.r per load case.  Of the 6 threads of a row, thread q
 // keeps load cases 4q..4q+3.
-__global__ __launch_bounds__(NT_UPD) void k_bupdate(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+__global__ __launch_bounds__(NT_UPD) void k_bupdate(const int4* __restrict__ slices, const int* __restrict__ done,
                                                    const float* __restrict__ coef, float* __restrict__ x,
                                                    float* __restrict__ r, const float* __restrict__ p,
                                                    const float* __restrict__ q, double* __restrict__ prr) {
     __shared__ float4 s_c[R * 6];
     __shared__ double s_a[NT_UPD][4];
-    const BrickInfo B = bricks[blockIdx.x];
-    if (done[B.dom]) return;
+    const int4 sl = slices[blockIdx.x];   // domain, first node, end node
+    if (done[sl.x]) return;
     const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
-    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)B.dom * RR);
+    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)sl.x * RR);
     for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
     __syncthreads();
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t r1 = (int64_t)B.node_end * 3;
-    for (int64_t row = (int64_t)B.node_begin * 3 + rw; row < r1; row += NT_UPD / 6) {
+    const int64_t r1 = (int64_t)sl.z * 3;
+    for (int64_t row = (int64_t)sl.y * 3 + rw; row < r1; row += NT_UPD / 6) {
         const float4* pr = reinterpret_cast<const float4*>(p + row * R);
         const float4* qr = reinterpret_cast<const float4*>(q + row * R);
         float4* xr = reinterpret_cast<float4*>(x + row * R) + qd;
@@ -1354,21 +1354,21 @@ __global__ __launch_bounds__(NT_UPD) void k_bupdate(const BrickInfo* __restrict_
 // The FP64 form of k_bupdate (block PCG on FP64 vectors): x += P alpha,
 // r -= Q alpha with x, r, Q in FP64 and the directions P in FP32, per-brick r.r.
 // Of the 6 threads of a row, thread q keeps load cases 4q..4q+3 (as k_bupdate).
-__global__ __launch_bounds__(NT_UPD) void k_bupdate64(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+__global__ __launch_bounds__(NT_UPD) void k_bupdate64(const int4* __restrict__ slices, const int* __restrict__ done,
                                                      const float* __restrict__ coef, double* __restrict__ x,
                                                      double* __restrict__ r, const float* __restrict__ p,
                                                      const double* __restrict__ q, double* __restrict__ prr) {
     __shared__ float4 s_c[R * 6];
     __shared__ double s_a[NT_UPD][4];
-    const BrickInfo B = bricks[blockIdx.x];
-    if (done[B.dom]) return;
+    const int4 sl = slices[blockIdx.x];   // domain, first node, end node
+    if (done[sl.x]) return;
     const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
-    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)B.dom * RR);
+    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)sl.x * RR);
     for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
     __syncthreads();
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t r1 = (int64_t)B.node_end * 3;
-    for (int64_t row = (int64_t)B.node_begin * 3 + rw; row < r1; row += NT_UPD / 6) {
+    const int64_t r1 = (int64_t)sl.z * 3;
+    for (int64_t row = (int64_t)sl.y * 3 + rw; row < r1; row += NT_UPD / 6) {
         const float4* pr = reinterpret_cast<const float4*>(p + row * R);
         const double2* qr = reinterpret_cast<const double2*>(q + row * R);
         double2* xr = reinterpret_cast<double2*>(x + row * R) + 2 * qd;
@@ -1414,18 +1414,18 @@ __global__ __launch_bounds__(NT_UPD) void k_bupdate64(const BrickInfo* __restric
 // pn = Z + P coef (z null: pn = P coef); pn aliases neither p nor z.  With beta
 // (k_bsolve mode 1) the new direction block, with L^-T (mode 2) its orthonormal
 // columns
-__global__ __launch_bounds__(NT_UPD) void k_bdir(const BrickInfo* __restrict__ bricks, const int* __restrict__ done,
+__global__ __launch_bounds__(NT_UPD) void k_bdir(const int4* __restrict__ slices, const int* __restrict__ done,
                                                 const float* __restrict__ coef, const float* __restrict__ p,
                                                 const float* __restrict__ z, float* __restrict__ pn) {
     __shared__ float4 s_c[R * 6];
-    const BrickInfo B = bricks[blockIdx.x];
-    if (done[B.dom]) return;
+    const int4 sl = slices[blockIdx.x];   // domain, first node, end node
+    if (done[sl.x]) return;
     const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
-    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)B.dom * RR);
+    const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)sl.x * RR);
     for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
     __syncthreads();
-    const int64_t r1 = (int64_t)B.node_end * 3;
-    for (int64_t row = (int64_t)B.node_begin * 3 + rw; row < r1; row += NT_UPD / 6) {
+    const int64_t r1 = (int64_t)sl.z * 3;
+    for (int64_t row = (int64_t)sl.y * 3 + rw; row < r1; row += NT_UPD / 6) {
         const float4* pr = reinterpret_cast<const float4*>(p + row * R);
         float4 v = z ? reinterpret_cast<const float4*>(z + row * R)[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -1445,16 +1445,15 @@ __global__ __launch_bounds__(NT_UPD) void k_bdir(const BrickInfo* __restrict__ b
 // per domain: the inner block iteration has finished when every load case's
 // recursive residual (scaled back by its column scale, see k_to32) is at or
 // below its target (a non-finite one is not), or at the iteration cap
-__global__ __launch_bounds__(RED_NT) void k_bcheck(const DomInfo* __restrict__ doms, int* __restrict__ done,
+__global__ __launch_bounds__(RED_NT) void k_bcheck(const int* __restrict__ dslice, int* __restrict__ done,
                                                   const double* __restrict__ prr, const double* __restrict__ thr2,
                                                   const double* __restrict__ rrcur, int64_t* __restrict__ iters,
                                                   int64_t max_iters) {
     __shared__ double s_out[4 * R];
     const int d = blockIdx.x, tid = threadIdx.x;
     if (done[d]) return;
-    const DomInfo D = doms[d];
     double rr[R];
-    dom_sum24(prr, D.brick_begin, D.brick_end, s_out, rr);
+    dom_sum24(prr, dslice[d], dslice[d + 1], s_out, rr);   // per-slice partials of k_bupdate
     if (tid == 0) {
         int any = 0;
         for (int lc = 0; lc < R; ++lc) {
@@ -3225,6 +3224,8 @@ int block_setup(SolverBatch& b) {
     DTC_TRY(b.gsum.ensure((size_t)std::max(b.n_dom, 1) * 2 * RR * 8));
     DTC_TRY(b.lfac.ensure((size_t)std::max(b.n_dom, 1) * RR * 8));
     DTC_TRY(b.bcoef.ensure((size_t)std::max(b.n_dom, 1) * RR * 4));
+    // per-slice r.r partials of the block update (fewer slices than bricks, but no assumption)
+    DTC_TRY(b.part_b.ensure(std::max<size_t>({sl.size(), (size_t)b.n_bricks, 1}) * R * 8));
     if (!sl.empty())
         DTC_HIP(hipMemcpyAsync(b.slices.p, sl.data(), sl.size() * sizeof(int4), hipMemcpyHostToDevice, b.stream));
     DTC_HIP(hipMemcpyAsync(b.dslice.p, ds.data(), ds.size() * 4, hipMemcpyHostToDevice, b.stream));
@@ -3271,7 +3272,7 @@ int block_orth(SolverBatch& b, const float* src, float* dst) {
     int* done = b.done.as<int>();
     DTC_TRY(block_gram(b, src, src, nullptr));
     k_bsolve<<<b.n_dom, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 2, b.lfac.as<double>(), b.bcoef.as<float>());
-    k_bdir<<<b.n_bricks, NT_UPD, 0, s>>>(b.brick.as<BrickInfo>(), done, b.bcoef.as<float>(), src, nullptr, dst);
+    k_bdir<<<b.n_slices, NT_UPD, 0, s>>>(b.slices.as<int4>(), done, b.bcoef.as<float>(), src, nullptr, dst);
     DTC_HIP(hipGetLastError());
     return DTC_OK;
 }
@@ -3284,7 +3285,8 @@ int block_orth(SolverBatch& b, const float* src, float* dst) {
 // stiff regions' terms.
 int block_iteration64(SolverBatch& b, const WK<float>& wk, float* p, float* q, float* z, int64_t max_iters) {
     hipStream_t s = b.stream;
-    const int D = b.n_dom, NBk = b.n_bricks;
+    const int D = b.n_dom, NBk = b.n_bricks, NSl = b.n_slices;
+    const int4* sl = b.slices.as<int4>();
     int* done = b.done.as<int>();
     const BrickInfo* bl = b.brick.as<BrickInfo>();
     double* x = b.x64.as<double>();
@@ -3294,15 +3296,15 @@ int block_iteration64(SolverBatch& b, const WK<float>& wk, float* p, float* q, f
     else DTC_TRY(launch_mv_q64(b, p, nullptr, done, q64));
     DTC_TRY((block_gram<float, double>(b, p, q64, r)));
     k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 0, b.lfac.as<double>(), b.bcoef.as<float>());
-    k_bupdate64<<<NBk, NT_UPD, 0, s>>>(bl, done, b.bcoef.as<float>(), x, r, p, q64, b.part_b.as<double>());
-    k_bcheck<<<D, RED_NT, 0, s>>>(b.dom.as<DomInfo>(), done, b.part_b.as<double>(), b.thr2.as<double>(), nullptr,
+    k_bupdate64<<<NSl, NT_UPD, 0, s>>>(sl, done, b.bcoef.as<float>(), x, r, p, q64, b.part_b.as<double>());
+    k_bcheck<<<D, RED_NT, 0, s>>>(b.dslice.as<int>(), done, b.part_b.as<double>(), b.thr2.as<double>(), nullptr,
                                   b.iters.as<int64_t>(), max_iters);
     k_to32<<<NBk, NT_UPD, 0, s>>>(bl, done, r, b.r32.as<float>(), nullptr, nullptr);
     DTC_HIP(hipGetLastError());
     DTC_TRY(mg_vcycle(b, wk, b.r32.as<float>(), z, nullptr));
     DTC_TRY((block_gram<double, float>(b, q64, z, nullptr)));
     k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 1, b.lfac.as<double>(), b.bcoef.as<float>());
-    k_bdir<<<NBk, NT_UPD, 0, s>>>(bl, done, b.bcoef.as<float>(), p, z, q);
+    k_bdir<<<NSl, NT_UPD, 0, s>>>(sl, done, b.bcoef.as<float>(), p, z, q);
     DTC_HIP(hipGetLastError());
     return block_orth(b, q, p);
 }
@@ -3313,22 +3315,22 @@ int block_iteration(SolverBatch& b, const WK<float>& wk, float* x, float* r, flo
                     int64_t max_iters) {
     if (b.block64) return block_iteration64(b, wk, p, q, z, max_iters);
     hipStream_t s = b.stream;
-    const int D = b.n_dom, NBk = b.n_bricks;
+    const int D = b.n_dom, NSl = b.n_slices;
+    const int4* sl = b.slices.as<int4>();
     int* done = b.done.as<int>();
-    const BrickInfo* bl = b.brick.as<BrickInfo>();
     if (b.block_q64 && b.micro20) DTC_TRY(mv20_q64(b, p, q, done));
     else if (b.block_q64) DTC_TRY(launch_mv_q64(b, p, q, done));
     else DTC_TRY((launch_mv<float, IN_V, OUT_MASKED>(b, wk, p, q, nullptr, done)));
     DTC_TRY(block_gram(b, p, q, r));
     k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 0, b.lfac.as<double>(), b.bcoef.as<float>());
-    k_bupdate<<<NBk, NT_UPD, 0, s>>>(bl, done, b.bcoef.as<float>(), x, r, p, q, b.part_b.as<double>());
-    k_bcheck<<<D, RED_NT, 0, s>>>(b.dom.as<DomInfo>(), done, b.part_b.as<double>(), b.thr2.as<double>(),
+    k_bupdate<<<NSl, NT_UPD, 0, s>>>(sl, done, b.bcoef.as<float>(), x, r, p, q, b.part_b.as<double>());
+    k_bcheck<<<D, RED_NT, 0, s>>>(b.dslice.as<int>(), done, b.part_b.as<double>(), b.thr2.as<double>(),
                                   b.rr.as<double>(), b.iters.as<int64_t>(), max_iters);
     DTC_HIP(hipGetLastError());
     DTC_TRY(mg_vcycle(b, wk, r, z, nullptr));
     DTC_TRY(block_gram(b, q, z, nullptr));
     k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 1, b.lfac.as<double>(), b.bcoef.as<float>());
-    k_bdir<<<NBk, NT_UPD, 0, s>>>(bl, done, b.bcoef.as<float>(), p, z, q);
+    k_bdir<<<NSl, NT_UPD, 0, s>>>(sl, done, b.bcoef.as<float>(), p, z, q);
     DTC_HIP(hipGetLastError());
     return block_orth(b, q, p);
 }
