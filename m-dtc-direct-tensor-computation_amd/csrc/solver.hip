@@ -3360,6 +3360,11 @@ __global__ __launch_bounds__(256) void k_all_set(const int* __restrict__ flags, 
 // a sequence of early exits (every kernel skips done domains), which leaves the
 // results unchanged.  DTC_GRAPH=0 (or kernel timing) launches the iterations one
 // by one with a synchronous check every GROUP iterations.
+inline bool trace_on() {
+    static const bool on = std::getenv("DTC_TRACE") && std::getenv("DTC_TRACE")[0] == '1';
+    return on;
+}
+
 int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p, float* q, float* z,
               int64_t max_iters, int64_t max_steps) {
     constexpr int GROUP = 4;
@@ -3431,6 +3436,9 @@ int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p,
             break;
         }
         if (b.all_done_host.p[(launched - 2) & 1]) break;
+        // DTC_TRACE=1: a progress line every 1,000 block iterations of a long phase
+        if (trace_on() && (steps + GROUP) % 1000 < GROUP)
+            std::fprintf(stderr, "[dtc] block phase: %lld iterations\n", (long long)(steps + GROUP));
     }
     const hipError_t es = hipStreamSynchronize(s);
     for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
