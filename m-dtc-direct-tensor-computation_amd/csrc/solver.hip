@@ -1308,23 +1308,43 @@ __global__ __launch_bounds__(NT_UPD) void k_bupdate(const int4* __restrict__ sli
                                                    const float* __restrict__ q, double* __restrict__ prr) {
     __shared__ float4 s_c[R * 6];
     __shared__ double s_a[NT_UPD][4];
+    __shared__ float4 s_pq[2][NT_UPD / 6][6];   // P and Q rows of the current block
     const int4 sl = slices[blockIdx.x];   // domain, first node, end node
     if (done[sl.x]) return;
     const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
     const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)sl.x * RR);
     for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
-    __syncthreads();
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t r1 = (int64_t)sl.z * 3;
-    for (int64_t row = (int64_t)sl.y * 3 + rw; row < r1; row += NT_UPD / 6) {
-        const float4* pr = reinterpret_cast<const float4*>(p + row * R);
-        const float4* qr = reinterpret_cast<const float4*>(q + row * R);
+    const int64_t r0 = (int64_t)sl.y * 3, r1 = (int64_t)sl.z * 3;
+    // the block's P and Q rows are loaded once (thread t: 16-byte vector t of P and
+    // of Q, coalesced) and read from LDS by the 6 threads of a row (a broadcast),
+    // instead of every thread loading both whole rows; the next block's vectors are
+    // in flight during the current block's products
+    constexpr int RB = NT_UPD / 6;   // rows per block (one per row lane)
+    float4 pf_p, pf_q;
+    auto load = [&](int64_t b0) {
+        const int64_t row = b0 + tid / 6;
+        pf_p = pf_q = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < r1) {
+            pf_p = reinterpret_cast<const float4*>(p + row * R)[qd];
+            pf_q = reinterpret_cast<const float4*>(q + row * R)[qd];
+        }
+    };
+    if (r0 < r1) load(r0);
+    for (int64_t b0 = r0; b0 < r1; b0 += RB) {
+        __syncthreads();                 // the previous block's reads of s_pq are done (and s_c is set)
+        s_pq[0][tid / 6][qd] = pf_p;
+        s_pq[1][tid / 6][qd] = pf_q;
+        __syncthreads();
+        if (b0 + RB < r1) load(b0 + RB);
+        const int64_t row = b0 + rw;
+        if (row >= r1) continue;
         float4* xr = reinterpret_cast<float4*>(x + row * R) + qd;
         float4* rr = reinterpret_cast<float4*>(r + row * R) + qd;
         float4 xv = *xr, rv = *rr;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            const float4 pv = pr[c], qv = qr[c];
+            const float4 pv = s_pq[0][rw][c], qv = s_pq[1][rw][c];
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
                 const float4 a = s_c[(4 * c + w) * 6 + qd];
@@ -1418,19 +1438,31 @@ __global__ __launch_bounds__(NT_UPD) void k_bdir(const int4* __restrict__ slices
                                                 const float* __restrict__ coef, const float* __restrict__ p,
                                                 const float* __restrict__ z, float* __restrict__ pn) {
     __shared__ float4 s_c[R * 6];
+    __shared__ float4 s_p[NT_UPD / 6][6];   // P rows of the current block (staged as in k_bupdate)
     const int4 sl = slices[blockIdx.x];   // domain, first node, end node
     if (done[sl.x]) return;
     const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
     const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)sl.x * RR);
     for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
-    __syncthreads();
-    const int64_t r1 = (int64_t)sl.z * 3;
-    for (int64_t row = (int64_t)sl.y * 3 + rw; row < r1; row += NT_UPD / 6) {
-        const float4* pr = reinterpret_cast<const float4*>(p + row * R);
+    constexpr int RB = NT_UPD / 6;
+    const int64_t r0 = (int64_t)sl.y * 3, r1 = (int64_t)sl.z * 3;
+    float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto load = [&](int64_t b0) {
+        const int64_t row = b0 + tid / 6;
+        pf = row < r1 ? reinterpret_cast<const float4*>(p + row * R)[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    if (r0 < r1) load(r0);
+    for (int64_t b0 = r0; b0 < r1; b0 += RB) {
+        __syncthreads();
+        s_p[tid / 6][qd] = pf;
+        __syncthreads();
+        if (b0 + RB < r1) load(b0 + RB);
+        const int64_t row = b0 + rw;
+        if (row >= r1) continue;
         float4 v = z ? reinterpret_cast<const float4*>(z + row * R)[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            const float4 pv = pr[c];
+            const float4 pv = s_p[rw][c];
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
                 const float4 a = s_c[(4 * c + w) * 6 + qd];
