@@ -1380,17 +1380,40 @@ __global__ __launch_bounds__(NT_UPD) void k_bupdate64(const int4* __restrict__ s
                                                      const double* __restrict__ q, double* __restrict__ prr) {
     __shared__ float4 s_c[R * 6];
     __shared__ double s_a[NT_UPD][4];
+    __shared__ float4 s_p[NT_UPD / 6][6];      // the block's P rows and
+    __shared__ double2 s_q[NT_UPD / 6][12];    // Q rows (staged as in k_bupdate)
     const int4 sl = slices[blockIdx.x];   // domain, first node, end node
     if (done[sl.x]) return;
     const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
     const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)sl.x * RR);
     for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
-    __syncthreads();
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t r1 = (int64_t)sl.z * 3;
-    for (int64_t row = (int64_t)sl.y * 3 + rw; row < r1; row += NT_UPD / 6) {
-        const float4* pr = reinterpret_cast<const float4*>(p + row * R);
-        const double2* qr = reinterpret_cast<const double2*>(q + row * R);
+    constexpr int RB = NT_UPD / 6;
+    const int64_t r0b = (int64_t)sl.y * 3, r1 = (int64_t)sl.z * 3;
+    float4 pf_p;
+    double2 pf_q[2];
+    auto load = [&](int64_t b0) {
+        const int64_t row = b0 + tid / 6;
+        pf_p = make_float4(0.f, 0.f, 0.f, 0.f);
+        pf_q[0] = pf_q[1] = make_double2(0.0, 0.0);
+        if (row < r1) {
+            pf_p = reinterpret_cast<const float4*>(p + row * R)[qd];
+            pf_q[0] = reinterpret_cast<const double2*>(q + row * R)[2 * qd];
+            pf_q[1] = reinterpret_cast<const double2*>(q + row * R)[2 * qd + 1];
+        }
+    };
+    if (r0b < r1) load(r0b);
+    for (int64_t b0 = r0b; b0 < r1; b0 += RB) {
+        __syncthreads();
+        s_p[tid / 6][qd] = pf_p;
+        s_q[tid / 6][2 * qd] = pf_q[0];
+        s_q[tid / 6][2 * qd + 1] = pf_q[1];
+        __syncthreads();
+        if (b0 + RB < r1) load(b0 + RB);
+        const int64_t row = b0 + rw;
+        if (row >= r1) continue;
+        const float4* pr = s_p[rw];
+        const double2* qr = s_q[rw];
         double2* xr = reinterpret_cast<double2*>(x + row * R) + 2 * qd;
         double2* rr = reinterpret_cast<double2*>(r + row * R) + 2 * qd;
         double xv[4], rv[4];
