@@ -2012,15 +2012,26 @@ __global__ __launch_bounds__(256) void k_restrict(const BrickInfo* __restrict__ 
                                                   const int* __restrict__ cl2n, const DomInfo* __restrict__ fdom,
                                                   const int* __restrict__ fl2n, const float* __restrict__ rf,
                                                   const int* __restrict__ done, float* __restrict__ bc) {
+    // fine lattice nodes of the brick's restriction stencils, staged once: positions
+    // 2 x0 - 1 .. 2 (x0 + BX - 1) + 1 (and likewise in y, z); -1 = no node
+    constexpr int FX = 2 * BX + 1, FY = 2 * BY + 1, FZ = 2 * BZ + 1;
     __shared__ unsigned char s_pos[NO];
+    __shared__ int s_f2n[FX * FY * FZ];
     const BrickInfo B = cbr[blockIdx.x];
     if (done && done[B.dom]) return;
     const int tid = threadIdx.x;
     const DomInfo C = cdom[B.dom], F = fdom[B.dom];
-    const int n = brick_positions(B, C, cl2n, s_pos);
+    const int fLx = F.nx + 1, fLy = F.ny + 1;
+    const int fx0 = 2 * B.x0 - 1, fy0 = 2 * B.y0 - 1, fz0 = 2 * B.z0 - 1;
+    for (int i = tid; i < FX * FY * FZ; i += 256) {
+        const int X = fx0 + i % FX, Y = fy0 + (i / FX) % FY, Z = fz0 + i / (FX * FY);
+        s_f2n[i] = (X >= 0 && Y >= 0 && Z >= 0 && X <= F.nx && Y <= F.ny && Z <= F.nz)
+                       ? fl2n[F.lat_off + X + (int64_t)fLx * (Y + (int64_t)fLy * Z)]
+                       : -1;
+    }
+    const int n = brick_positions(B, C, cl2n, s_pos);   // (its barrier also publishes s_f2n)
     if (tid >= T_NPP * T_LPN) return;
     const int slot = tid / T_LPN, j = tid % T_LPN;
-    const int fLx = F.nx + 1, fLy = F.ny + 1;
     const float4* rf4 = reinterpret_cast<const float4*>(rf);
     float4* bc4 = reinterpret_cast<float4*>(bc);
     for (int k = slot + (int)blockIdx.y * T_NPP; k < n; k += T_NPP * (int)gridDim.y) {
@@ -2036,7 +2047,7 @@ __global__ __launch_bounds__(256) void k_restrict(const BrickInfo* __restrict__ 
                     for (int i = -1; i <= 1; ++i) {
                         const int X = 2 * gx + i, Y = 2 * gy + jj, Z = 2 * gz + k;
                         if (X > 0 && Y > 0 && Z > 0 && X < F.nx && Y < F.ny && Z < F.nz) {   // free fine node
-                            const int qf = fl2n[F.lat_off + X + (int64_t)fLx * (Y + (int64_t)fLy * Z)];
+                            const int qf = s_f2n[(X - fx0) + FX * ((Y - fy0) + FY * (Z - fz0))];
                             if (qf >= 0) {
                                 constexpr float W[3] = {0.5f, 1.f, 0.5f};
                                 const float w = W[i + 1] * W[jj + 1] * W[k + 1];
@@ -2055,15 +2066,26 @@ __global__ __launch_bounds__(256) void k_prolong(const BrickInfo* __restrict__ f
                                                  const int* __restrict__ fl2n, const DomInfo* __restrict__ cdom,
                                                  const int* __restrict__ cl2n, const float* __restrict__ xc,
                                                  const int* __restrict__ done, float* __restrict__ xf) {
+    // coarse lattice nodes the brick's fine nodes interpolate from, staged once:
+    // positions x0 / 2 .. (x0 + BX - 1) / 2 + 1 (and likewise in y, z); -1 = no node
+    constexpr int CXM = BX / 2 + 2, CYM = BY / 2 + 2, CZM = BZ / 2 + 2;   // room for odd origins
     __shared__ unsigned char s_pos[NO];
+    __shared__ int s_c2n[CXM * CYM * CZM];
     const BrickInfo B = fbr[blockIdx.x];
     if (done && done[B.dom]) return;
     const int tid = threadIdx.x;
     const DomInfo F = fdom[B.dom], C = cdom[B.dom];
-    const int n = brick_positions(B, F, fl2n, s_pos);
+    const int cLx = C.nx + 1, cLy = C.ny + 1;
+    const int cx0 = B.x0 >> 1, cy0 = B.y0 >> 1, cz0 = B.z0 >> 1;
+    for (int i = tid; i < CXM * CYM * CZM; i += 256) {
+        const int X = cx0 + i % CXM, Y = cy0 + (i / CXM) % CYM, Z = cz0 + i / (CXM * CYM);
+        s_c2n[i] = (X <= C.nx && Y <= C.ny && Z <= C.nz)
+                       ? cl2n[C.lat_off + X + (int64_t)cLx * (Y + (int64_t)cLy * Z)]
+                       : -1;
+    }
+    const int n = brick_positions(B, F, fl2n, s_pos);   // (its barrier also publishes s_c2n)
     if (tid >= T_NPP * T_LPN) return;
     const int slot = tid / T_LPN, j = tid % T_LPN;
-    const int cLx = C.nx + 1, cLy = C.ny + 1;
     const float4* xc4 = reinterpret_cast<const float4*>(xc);
     float4* xf4 = reinterpret_cast<float4*>(xf);
     for (int k = slot + (int)blockIdx.y * T_NPP; k < n; k += T_NPP * (int)gridDim.y) {
@@ -2081,7 +2103,7 @@ __global__ __launch_bounds__(256) void k_prolong(const BrickInfo* __restrict__ f
 #pragma unroll
                 for (int i = 0; i < 2; ++i) {
                     if ((i && !ox) || (jj && !oy) || (k && !oz)) continue;
-                    const int qc = cl2n[C.lat_off + (X0 + i) + (int64_t)cLx * ((Y0 + jj) + (int64_t)cLy * (Z0 + k))];
+                    const int qc = s_c2n[(X0 + i - cx0) + CXM * ((Y0 + jj - cy0) + CYM * (Z0 + k - cz0))];
                     if (qc >= 0) f4_axpy(sum, 1.f, xc4[(int64_t)qc * T_LPN + j]);
                 }
         float4 v = xf4[(int64_t)qf * T_LPN + j];
