@@ -46,7 +46,9 @@ namespace {
 constexpr int R = R24;
 constexpr int NT_UPD = 192;    // multiple of 24: a thread keeps one load case
 
-enum InMode { IN_V = 0, IN_U = 1 };
+// IN_JS (FP32 Chebyshev step only): x = cin D^-1 v, the first smoothing step from
+// x = 0 formed while staging v, so that it is never stored as a vector
+enum InMode { IN_V = 0, IN_U = 1, IN_JS = 2 };
 enum OutMode { OUT_MASKED_DOT = 0, OUT_RESID = 1, OUT_PLAIN = 2, OUT_MASKED = 3, OUT_CHEB = 4, OUT_BRES = 5 };
 
 template <typename T> struct WK { T g[45]; };
@@ -231,6 +233,7 @@ struct MvArgs {
     void* dvec;                 // Chebyshev direction
     float c1, c2;               // Chebyshev coefficients of this step
     int cflags;                 // CH_* : where the previous direction comes from, whether d is stored
+    float cin = 0.f;            // IN_JS: the zero-start step's coefficient
 };
 // OUT_CHEB direction handling: the previous direction d is read from dvec
 // (CH_READ_D), or equals the current iterate (CH_D_FROM_X: the step after a
@@ -252,6 +255,12 @@ constexpr int CH_READ_D = 1, CH_D_FROM_X = 2, CH_WRITE_D = 4;
 //   gather  each owned (node, direction) sums its corner buffers in corner
 //           order (deterministic) and is written once.
 // OUT_MASKED_DOT zeroes Dirichlet rows and writes per-brick sums of x.y;
+// the zero-start smoothing step fused into the next step (DTC_CHEB_FUSE=0: stored by k_cheb)
+inline bool cheb_fuse() {
+    const char* e = std::getenv("DTC_CHEB_FUSE");
+    return !(e && e[0] == '0');
+}
+
 // whole-segment output stores of the FP32 matvec (DTC_MV_DEFER=0: per-chunk stores)
 inline bool mv_defer() {
     const char* e = std::getenv("DTC_MV_DEFER");
@@ -298,6 +307,7 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
     __shared__ int s_wcnt[((NH + NT - 1) / NT) * NW];
     __shared__ short s_sb[SPARSE ? NO : 1];      // SPARSE: first slot of each owned position
     static_assert(!IO32 || (sizeof(T) == 8 && IN == IN_V), "FP32 vectors with FP64 arithmetic");
+    static_assert(IN != IN_JS || (sizeof(T) == 4 && OUT == OUT_CHEB && !SPARSE), "zero-start input: FP32 Chebyshev");
 
     const int bid = A.blist ? A.blist[A.boff + blockIdx.x] : (int)blockIdx.x;
     const BrickInfo B = A.bricks[bid];
@@ -322,6 +332,7 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
     int nh = NH;
     int pf_off[NLD], pf_xi[NLD];   // 16-byte offset in x and destination in s_x of this thread's items
     V reg[NLD];                    // x segments of the next chunk to stage, loaded one chunk ahead
+    T pf_sc[IN == IN_JS ? NLD : 1]; // IN_JS: cin times the item's Jacobi scaling
     auto prefetch = [&](int chunk) {
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
@@ -350,7 +361,7 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
             pf_off[k] = o;
             pf_xi[k] = i;
         }
-        if (IN == IN_V) prefetch(c0);
+        if (IN != IN_U) prefetch(c0);
     }
     {
         // halo positions h = tid + k NT (one pass with 512 threads, two with 256)
@@ -431,13 +442,25 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
             }
             pf_off[k] = o;
             pf_xi[k] = xi;
+            if constexpr (IN == IN_JS)   // the product order of k_cheb's zero start: (c2 dinv) v
+                pf_sc[k] = o >= 0 ? (T)A.cin * reinterpret_cast<const T*>(A.dinv)[o / NCH] : T(0);
         }
-        if (IN == IN_V) prefetch(c0);
+        if (IN != IN_U) prefetch(c0);
     }
 
     // s_x <- x of `chunk` (IN_V: from the prefetch registers; IN_U: x + BC values)
     auto stage = [&](int chunk) {
-        if (IN == IN_V) {
+        if constexpr (IN == IN_JS) {
+#pragma unroll
+            for (int k = 0; k < NLD; ++k)
+                if (pf_off[k] >= 0) {
+                    V v = reg[k];
+                    T* vp = reinterpret_cast<T*>(&v);
+#pragma unroll
+                    for (int w = 0; w < RC; ++w) vp[w] = pf_sc[k] * vp[w];
+                    s_x[pf_xi[k]] = v;
+                }
+        } else if (IN == IN_V) {
 #pragma unroll
             for (int k = 0; k < NLD; ++k)
                 if (pf_off[k] >= 0) s_x[pf_xi[k]] = reg[k];
@@ -498,7 +521,7 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
             if (!((s_nm[o] >> b) & 1)) vzero(s_c[(OB_BIAS + b * NO + o) * 3 + d]);
         }
     stage(c0);
-    if (IN == IN_V && c0 + 1 < c1) prefetch(c0 + 1);
+    if (IN != IN_U && c0 + 1 < c1) prefetch(c0 + 1);
     __syncthreads();
 
     // FP32 dense Chebyshev step (one gather item per thread): the outputs of all
@@ -602,7 +625,7 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
         __syncthreads();
         if (chunk + 1 < c1) {
             stage(chunk + 1);
-            if (IN == IN_V && chunk + 2 < c1) prefetch(chunk + 2);
+            if (IN != IN_U && chunk + 2 < c1) prefetch(chunk + 2);
         }
 
         // ---- gather: thread per owned (node, direction), all RC load cases in one vector
@@ -1780,15 +1803,27 @@ __global__ __launch_bounds__(NT_UPD) void k_project(const BrickInfo* __restrict_
     }
 }
 
-__global__ void k_red_ff(const DomInfo* __restrict__ doms, const double* __restrict__ pff, double* __restrict__ ff) {
+// ff[d] = sum over the domain's bricks of the force partials: workgroup (d, g) takes
+// entries 32 g .. 32 g + 31 of the R x FFD block; its 8 thread rows sum every 8th
+// brick, and the 8 row sums are added in row order (fixed order)
+constexpr int RFF_E = 32, RFF_P = 8;   // entries per workgroup, brick partitions
+__global__ __launch_bounds__(RFF_E * RFF_P) void k_red_ff(const DomInfo* __restrict__ doms,
+                                                         const double* __restrict__ pff, double* __restrict__ ff) {
+    __shared__ double s_p[RFF_P][RFF_E];
     const int d = blockIdx.x;
     const DomInfo D = doms[d];
     const int FD = 3 * macro_nodes(D.macro_order);
-    for (int i = threadIdx.x; i < R * FFD; i += blockDim.x) {
-        double v = 0.0;
-        if (i % FFD < FD)
-            for (int b = D.brick_begin; b < D.brick_end; ++b) v += pff[(int64_t)b * R * FFD + i];
-        ff[(int64_t)d * R * FFD + i] = v;
+    const int e = threadIdx.x % RFF_E, part = threadIdx.x / RFF_E;
+    const int i = (int)blockIdx.y * RFF_E + e;
+    double v = 0.0;
+    if (i < R * FFD && i % FFD < FD)
+        for (int b = D.brick_begin + part; b < D.brick_end; b += RFF_P) v += pff[(int64_t)b * R * FFD + i];
+    s_p[part][e] = v;
+    __syncthreads();
+    if (part == 0 && i < R * FFD) {
+        double t = 0.0;
+        for (int k = 0; k < RFF_P; ++k) t += s_p[k][e];
+        ff[(int64_t)d * R * FFD + i] = t;
     }
 }
 
@@ -2952,12 +2987,17 @@ struct MfUse {
 
 // tb: the batch whose fine level this is (its launches are timed), else null
 template <int OUT>
+// cin != 0 (OUT_CHEB, dense bricks, VALU element products only): x is cin D^-1 x
+// (the zero-start step formed on the fly, IN_JS)
 int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float* x, float* y, const int* done,
                 const float* bv = nullptr, const float* dinv = nullptr, float* dv = nullptr, float c1 = 0.f,
                 float c2 = 0.f, double* partial = nullptr, int cflags = CH_READ_D | CH_WRITE_D,
-                SolverBatch* tb = nullptr, MfUse mf = MfUse{0, nullptr, 0.f}, int split = 0) {
+                SolverBatch* tb = nullptr, MfUse mf = MfUse{0, nullptr, 0.f}, int split = 0, float cin = 0.f) {
     if (T.n_bricks == 0) return DTC_OK;
+    if (cin != 0.f && (OUT != OUT_CHEB || mf.terms != 0 || T.n_sparse > 0))
+        return set_error(DTC_ERR_BAD_ARGUMENT, "zero-start input needs the dense FP32 Chebyshev step");
     MvArgs a{T.bricks, nullptr, 0, T.doms, T.halo, T.nmask, T.einfo, T.eE32, done, partial, bv, dinv, dv, c1, c2, cflags};
+    a.cin = cin;
     const int tk = OUT == OUT_CHEB ? TK_CHEB : TK_BRES;
     const dim3 grid(T.n_bricks, split > 0 ? split : tuned_split(tk, mf.terms, T.n_bricks, R / MvCfg<float>::RC));
     constexpr int kind = OUT == OUT_CHEB ? DTC_KSTAT_CHEB : OUT == OUT_BRES ? DTC_KSTAT_BRES : -1;
@@ -2982,10 +3022,24 @@ int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float
         if (T.n_bricks > T.n_sparse) {
             const dim3 gd(T.n_bricks - T.n_sparse, grid.y);
             const bool df = mv_defer();
-            if (CP && partial && df) k_matvec<float, IN_V, OUT, false, false, CP><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
+            bool js = false;
+            if constexpr (OUT == OUT_CHEB) {
+                js = cin != 0.f;
+                if (js && CP && partial && df) k_matvec<float, IN_JS, OUT, false, false, CP><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
+                else if (js && CP && partial) k_matvec<float, IN_JS, OUT, false, false, CP, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
+                else if (js && df) k_matvec<float, IN_JS, OUT, false, false, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
+                else if (js) k_matvec<float, IN_JS, OUT, false, false, false, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
+            }
+            if (js) {
+            } else if (CP && partial && df) k_matvec<float, IN_V, OUT, false, false, CP><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
             else if (CP && partial) k_matvec<float, IN_V, OUT, false, false, CP, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
             else if (df) k_matvec<float, IN_V, OUT, false, false, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
             else k_matvec<float, IN_V, OUT, false, false, false, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
+        }
+    } else if (cin != 0.f) {
+        if constexpr (OUT == OUT_CHEB) {
+            if (partial) k_matvec<float, IN_JS, OUT, false, false, true><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
+            else k_matvec<float, IN_JS, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
         }
     } else if (OUT == OUT_CHEB && partial) {
         k_matvec<float, IN_V, OUT, false, false, OUT == OUT_CHEB><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
@@ -3076,6 +3130,7 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
         const double hi = b.cheb_lmax * P.lscale, lo = hi / ratio;
         const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
         double rho = 1.0 / sigma;
+        float c0 = 0.f;   // the zero-start step's coefficient, when that step is fused into the next
         for (int k = 0; k < m; ++k) {
             float c1 = 0.f, c2 = (float)(1.0 / theta);
             if (k > 0) {
@@ -3085,15 +3140,22 @@ int mg_vcycle(SolverBatch& b, const WK<float>& wk, const float* r, float* z, dou
                 rho = rn;
             }
             double* dk = k == m - 1 ? dot : nullptr;
-            if (zero && k == 0) {
+            // the zero-start step x_1 = c2 D^-1 b is formed inside the next step's
+            // staging (IN_JS) instead of being stored, where that kernel form runs
+            const bool fuse = zero && m >= 2 && mfu.terms == 0 && v.T.n_sparse == 0 && cheb_fuse();
+            if (zero && k == 0 && fuse) {
+                c0 = c2;
+            } else if (zero && k == 0) {
                 // x_1 = d_0: the next step takes its previous direction from x
                 k_cheb<<<v.T.n_bricks, NT_UPD, 0, s>>>(v.T.bricks, done, v.b, nullptr, v.dinv, nullptr, v.buf[cur[l]],
                                                        c1, c2, 1, dk);
                 DTC_HIP(hipGetLastError());
             } else {
                 const int fl = (k == 0 ? 0 : (zero && k == 1) ? CH_D_FROM_X : CH_READ_D) | (k < m - 1 ? CH_WRITE_D : 0);
-                DTC_TRY(launch_mv32<OUT_CHEB>(v.T, s, wk, v.buf[cur[l]], v.buf[1 - cur[l]], done, v.b, v.dinv, v.d,
-                                              c1, c2, dk, fl, l == 0 ? &b : nullptr, mfu));
+                const bool js = fuse && k == 1;
+                DTC_TRY(launch_mv32<OUT_CHEB>(v.T, s, wk, js ? v.b : v.buf[cur[l]], v.buf[1 - cur[l]], done, v.b,
+                                              v.dinv, v.d, c1, c2, dk, fl, l == 0 ? &b : nullptr, mfu, 0,
+                                              js ? c0 : 0.f));
                 cur[l] = 1 - cur[l];
             }
         }
@@ -3926,7 +3988,8 @@ int batch_forces(SolverBatch& b, std::vector<double>& ff) {
                                                            b.lat2node.as<int>(), b.r64.as<double>(),
                                                            b.part_ff.as<double>());
     }
-    k_red_ff<<<D, 256, 0, b.stream>>>(b.dom.as<DomInfo>(), b.part_ff.as<double>(), b.ff.as<double>());
+    k_red_ff<<<dim3(D, (R * FFD + RFF_E - 1) / RFF_E), RFF_E * RFF_P, 0, b.stream>>>(b.dom.as<DomInfo>(),
+                                                                                b.part_ff.as<double>(), b.ff.as<double>());
     DTC_HIP(hipGetLastError());
     DTC_HIP(hipMemcpyAsync(ff.data(), b.ff.p, ff.size() * 8, hipMemcpyDeviceToHost, b.stream));
     DTC_HIP(hipStreamSynchronize(b.stream));

@@ -238,3 +238,22 @@ def test_lds_staged_gram_sums_equal_the_global_form(monkeypatch):
             assert a["iterations"] == c["iterations"]
             for k in ("C", "K_macro", "ff"):
                 assert np.array_equal(a[k], c[k]), k
+
+
+def test_fused_zero_start_smoothing_equals_the_stored_form(monkeypatch):
+    """The first Chebyshev step of each pre-smoothing pass (from x = 0: x1 = c D^-1 b)
+    is formed inside the next step's staging instead of being stored by a vector
+    kernel; the values are the same products, so the solves agree bit for bit with
+    DTC_CHEB_FUSE=0 (binary and greyscale batches)."""
+    dtc = _dtc()
+    for doms in ([_binary(dtc, 24, 91), _binary(dtc, 28, 92)], [_grey(dtc, 20, 93)]):
+        kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+        monkeypatch.setenv("DTC_CHEB_FUSE", "0")
+        ra = dtc.solve_batch(doms, **kw)
+        monkeypatch.delenv("DTC_CHEB_FUSE")
+        rb = dtc.solve_batch(doms, **kw)
+        for a, c in zip(ra, rb):
+            assert a["status"] == c["status"] == 0
+            assert a["iterations"] == c["iterations"]
+            for k in ("C", "K_macro", "ff"):
+                assert np.array_equal(a[k], c[k]), k
