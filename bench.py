@@ -274,8 +274,8 @@ def main():
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(domains shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one GPU is measured; --gpus {args.gpus} is not supported (domains would shard "
+              f"across GPUs, one process per GPU, with no exchange during the solves)", file=sys.stderr)
         sys.exit(2)
 
     import dtc_amd
