@@ -3537,12 +3537,14 @@ int block_run(SolverBatch& b, const WK<float>& wk, float* x, float* r, float* p,
         for (int k = 0; k < GROUP && rc == DTC_OK; ++k) rc = block_iteration(b, wk, x, r, p, q, z, max_iters);
         if (rc == DTC_OK) k_all_set<<<1, 256, 0, s>>>(b.done.as<int>(), b.n_dom, b.all_done_dev.as<int>());
         const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc != DTC_OK && g) (void)hipGraphDestroy(g);
         DTC_TRY(rc);
         if (ec != hipSuccess) return set_error(DTC_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ec));
         const hipError_t ei = hipGraphInstantiate(&b.block_graph.e, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
+        b.block_graph.g = g;
         if (ei != hipSuccess) {
             b.block_graph.e = nullptr;
+            b.block_graph.reset();
             return set_error(DTC_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
         }
         for (int k = 0; k < 6; ++k) b.graph_key[k] = key[k];

@@ -50,16 +50,21 @@ struct Dom20Dev {
     int Lx, Ly, Lz;
 };
 
-// An instantiated HIP graph (destroyed with its owner).
+// An instantiated HIP graph and the captured graph it was made from (both destroyed
+// with their owner; the captured graph is kept while its instance lives, since
+// tools that follow graph launches, such as a kernel-trace profiler, look at it)
 struct GraphExec {
     hipGraphExec_t e = nullptr;
+    hipGraph_t g = nullptr;
     GraphExec() = default;
     GraphExec(const GraphExec&) = delete;
     GraphExec& operator=(const GraphExec&) = delete;
     ~GraphExec() { reset(); }
     void reset() {
         if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
         e = nullptr;
+        g = nullptr;
     }
 };
 
