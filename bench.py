@@ -36,7 +36,7 @@ for p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # the timed kernels (fine-level launches): kernel_stats name -> kernel
 KERNEL_NAMES = {"mv": "k_matvec<float, IN_V, OUT_MASKED> (Q = K P of the block PCG)",
-                "cheb": "k_matvec<float, IN_V, OUT_CHEB> (Chebyshev-Jacobi smoothing step)",
+                "cheb": "k_matvec<float, IN_V|IN_JS, OUT_CHEB> (Chebyshev-Jacobi smoothing step)",
                 "bres": "k_matvec<float, IN_V, OUT_BRES> (b - K x before restriction)",
                 "resid": "k_matvec<double, IN_U, OUT_RESID> (FP64 true residual)"}
 

@@ -15,6 +15,7 @@ residual (bres) and the FP64 residual (resid).  Prints one JSON object.
 from __future__ import annotations
 
 import argparse
+import re
 import glob
 import json
 import os
@@ -55,7 +56,7 @@ def main() -> None:
             rows += c.execute(q).fetchall()
             c.close()
     for key, (kname, label) in KERNELS.items():
-        mine = [r for r in rows if kname in r[0].replace("(anonymous namespace)::", "")]
+        mine = [r for r in rows if re.search(kname, r[0].replace("(anonymous namespace)::", ""))]
         if not mine:
             continue
         variants = {}

@@ -21,6 +21,7 @@ domain edge match.
 from __future__ import annotations
 
 import argparse
+import re
 import glob
 import json
 import os
@@ -33,11 +34,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # bench kernel key -> (kernel name prefix as rocprofv3 writes it, the name bench.py reports).
 # A fine-level FP32 matvec is one launch of the sparse-brick variant and one of the
 # dense variant (template flags after the prefix): their per-dispatch averages add.
+# name pattern (regular expression) of each timed kernel, and its label; the
+# Chebyshev step runs with a vector input (IN_V = 0) or, after a zero start, with
+# the scaled right-hand side (IN_JS = 2): both are that kernel's launches
 KERNELS = {
-    "mv": ("k_matvec<float, 0, 3,", "k_matvec<float, IN_V, OUT_MASKED>"),
-    "cheb": ("k_matvec<float, 0, 4,", "k_matvec<float, IN_V, OUT_CHEB>"),
-    "bres": ("k_matvec<float, 0, 5,", "k_matvec<float, IN_V, OUT_BRES>"),
-    "resid": ("k_matvec<double, 1, 1,", "k_matvec<double, IN_U, OUT_RESID>"),
+    "mv": (r"k_matvec<float, 0, 3,", "k_matvec<float, IN_V, OUT_MASKED>"),
+    "cheb": (r"k_matvec<float, [02], 4,", "k_matvec<float, IN_V|IN_JS, OUT_CHEB>"),
+    "bres": (r"k_matvec<float, 0, 5,", "k_matvec<float, IN_V, OUT_BRES>"),
+    "resid": (r"k_matvec<double, 1, 1,", "k_matvec<double, IN_U, OUT_RESID>"),
 }
 COPY = ("import torch; a = torch.ones(2**28, device='cuda'); b = torch.empty_like(a); torch.cuda.synchronize(); "
         "b.copy_(a); torch.cuda.synchronize(); print(float(b[-1]))")
@@ -58,16 +62,17 @@ def _rows(dbdir: str, counter: str):
 
 
 def _counter(dbdir: str, counter: str, kname: str) -> tuple[float, int]:
-    """(mean value per fine-level call, dispatches): for every kernel variant whose
-    name starts with kname, the mean over its fine-level dispatches; the variants'
-    means are added (one dispatch of each per call)."""
-    rows = [r for r in _rows(dbdir, counter) if kname in r[0].replace("(anonymous namespace)::", "")]
+    """(mean value per fine-level dispatch, dispatches) over every kernel variant
+    whose name matches the pattern kname: for each variant its fine-level
+    dispatches, and the mean over all of them (the variants are separate launches)."""
+    pat = re.compile(kname)
+    rows = [r for r in _rows(dbdir, counter) if pat.search(r[0].replace("(anonymous namespace)::", ""))]
     if not rows:
         raise RuntimeError(f"no {counter} samples of {kname} in {dbdir}")
     by = {}
     for r in rows:
         by.setdefault(r[0], []).append(r)
-    total, n = 0.0, 0
+    vals = []
     for name, rs in by.items():
         if all(r[1] is not None for r in rs):
             cnt = {}
@@ -75,9 +80,8 @@ def _counter(dbdir: str, counter: str, kname: str) -> tuple[float, int]:
                 cnt[r[1]] = cnt.get(r[1], 0) + 1
             g = max(cnt, key=lambda k: (cnt[k], k))   # most dispatched grid, the larger on a tie
             rs = [r for r in rs if r[1] == g]
-        total += sum(r[2] for r in rs) / len(rs)
-        n = max(n, len(rs))
-    return total, n
+        vals += [r[2] for r in rs]
+    return sum(vals) / len(vals), len(vals)
 
 
 def _pmc(counter: str, out: str, cmd: list[str]) -> str:

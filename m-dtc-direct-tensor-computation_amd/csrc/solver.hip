@@ -3048,8 +3048,9 @@ int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float
     }
     DTC_HIP(hipGetLastError());
     if (tb) {
-        // x, y, b; Chebyshev: the direction read and written as the flags say
-        const int vp = 3 + (OUT == OUT_CHEB && (cflags & CH_READ_D) ? 1 : 0) +
+        // x, y, b (with the zero-start input x is b itself: 2 vectors); Chebyshev: the
+        // direction read and written as the flags say
+        const int vp = (cin != 0.f ? 2 : 3) + (OUT == OUT_CHEB && (cflags & CH_READ_D) ? 1 : 0) +
                        (OUT == OUT_CHEB && (cflags & CH_WRITE_D) ? 1 : 0);
         DTC_TRY(timed_end(*tb, slot, vp, false, OUT == OUT_CHEB, done));
     }
