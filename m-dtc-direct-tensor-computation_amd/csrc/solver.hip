@@ -2585,8 +2585,15 @@ __global__ __launch_bounds__(256) void k_brick_slots(const unsigned char* __rest
 // bricks of a level for the FP32 matvecs: those with at most MV_SLOTS corner
 // slots first (the sparse kernel), then the rest (the dense kernel)
 int brick_lists(hipStream_t s, int nbr, const DevMem& nmask, DevMem& blist, int* n_sparse) {
+    // DTC_SPARSE=1: bricks with few corner slots run the 256-thread variant (measured
+    // 10% slower over the whole mixed solve on trabecular batches, so not the default);
+    // otherwise no list is kept (brick = workgroup)
     *n_sparse = 0;
-    if (nbr == 0) return DTC_OK;
+    const char* se = std::getenv("DTC_SPARSE");
+    if (nbr == 0 || !(se && se[0] == '1')) {
+        blist.release();
+        return DTC_OK;
+    }
     DevMem cnt;
     DTC_TRY(cnt.alloc((size_t)nbr * 4));
     k_brick_slots<<<(nbr + 255) / 256, 256, 0, s>>>(nmask.as<unsigned char>(), nbr, cnt.as<int>());
@@ -2594,16 +2601,12 @@ int brick_lists(hipStream_t s, int nbr, const DevMem& nmask, DevMem& blist, int*
     std::vector<int> hc(nbr), order;
     DTC_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)nbr * 4, hipMemcpyDeviceToHost, s));
     DTC_HIP(hipStreamSynchronize(s));
-    // DTC_SPARSE=1: bricks with few corner slots run the 256-thread variant (measured
-    // 10% slower over the whole mixed solve on trabecular batches, so not the default)
-    const char* se = std::getenv("DTC_SPARSE");
-    const bool sparse = se && se[0] == '1';
     order.reserve(nbr);
     for (int i = 0; i < nbr; ++i)
-        if (sparse && hc[i] <= MV_SLOTS) order.push_back(i);
+        if (hc[i] <= MV_SLOTS) order.push_back(i);
     *n_sparse = (int)order.size();
     for (int i = 0; i < nbr; ++i)
-        if (!(sparse && hc[i] <= MV_SLOTS)) order.push_back(i);
+        if (hc[i] > MV_SLOTS) order.push_back(i);
     DTC_TRY(blist.ensure((size_t)nbr * 4));
     DTC_HIP(hipMemcpyAsync(blist.p, order.data(), (size_t)nbr * 4, hipMemcpyHostToDevice, s));
     DTC_HIP(hipStreamSynchronize(s));
@@ -2949,8 +2952,11 @@ int launch_mv(SolverBatch& b, const WK<T>& wk, const T* xin, T* yout, double* pa
                     if (mv_defer()) k_matvec<T, IN, OUT, false, false><<<gd, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
                     else k_matvec<T, IN, OUT, false, false, false, false><<<gd, MV_NT, 0, b.stream>>>(a, wk, xin, yout);
                 }
-            } else {
+            } else if (mv_defer()) {
                 k_matvec<T, IN, OUT><<<dim3(b.n_bricks, gy), MV_NT, 0, b.stream>>>(a, wk, xin, yout);
+            } else {
+                k_matvec<T, IN, OUT, false, false, false, false><<<dim3(b.n_bricks, gy), MV_NT, 0, b.stream>>>(a, wk, xin,
+                                                                                                         yout);
             }
         }
         DTC_HIP(hipGetLastError());
@@ -2984,6 +2990,14 @@ struct MfUse {
     const uint4* tab;
     float kmax;
 };
+
+// the dense FP32 Chebyshev step without a brick list: whole-segment stores unless
+// DTC_MV_DEFER=0 (df false)
+template <int IN, int OUT, bool PART>
+void mv32_dense(bool df, dim3 grid, hipStream_t s, const MvArgs& a, const WK<float>& wk, const float* x, float* y) {
+    if (df) k_matvec<float, IN, OUT, 0, false, PART, true><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
+    else k_matvec<float, IN, OUT, 0, false, PART, false><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
+}
 
 // tb: the batch whose fine level this is (its launches are timed), else null
 template <int OUT>
@@ -3036,13 +3050,12 @@ int launch_mv32(const MvTopo& T, hipStream_t s, const WK<float>& wk, const float
             else if (df) k_matvec<float, IN_V, OUT, false, false, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
             else k_matvec<float, IN_V, OUT, false, false, false, false><<<gd, MV_NT, 0, s>>>(as, wk, x, y);
         }
-    } else if (cin != 0.f) {
-        if constexpr (OUT == OUT_CHEB) {
-            if (partial) k_matvec<float, IN_JS, OUT, false, false, true><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
-            else k_matvec<float, IN_JS, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
-        }
-    } else if (OUT == OUT_CHEB && partial) {
-        k_matvec<float, IN_V, OUT, false, false, OUT == OUT_CHEB><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
+    } else if constexpr (OUT == OUT_CHEB) {
+        const bool df = mv_defer();
+        if (cin != 0.f && partial) mv32_dense<IN_JS, OUT, true>(df, grid, s, a, wk, x, y);
+        else if (cin != 0.f) mv32_dense<IN_JS, OUT, false>(df, grid, s, a, wk, x, y);
+        else if (partial) mv32_dense<IN_V, OUT, true>(df, grid, s, a, wk, x, y);
+        else mv32_dense<IN_V, OUT, false>(df, grid, s, a, wk, x, y);
     } else {
         k_matvec<float, IN_V, OUT><<<grid, MV_NT, 0, s>>>(a, wk, x, y);
     }
@@ -3714,6 +3727,9 @@ int autotune_splits(SolverBatch& b, const WK<float>& wk) {
                 rc = set_error(DTC_ERR_DEVICE, "autotune timing");
                 return;
             }
+            if (log)
+                std::fprintf(stderr, "dtc autotune:   %s level %d candidate gridDim.y %d: %.1f us\n", what, level, c,
+                             1e3f * ms / 3.f);
             if (ms < best_ms) { best_ms = ms; best = c; }
         }
         if (log)
