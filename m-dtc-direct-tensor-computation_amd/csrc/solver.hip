@@ -310,10 +310,33 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
     static_assert(IN != IN_JS || (sizeof(T) == 4 && OUT == OUT_CHEB && !SPARSE), "zero-start input: FP32 Chebyshev");
 
     const int bid = A.blist ? A.blist[A.boff + blockIdx.x] : (int)blockIdx.x;
-    const BrickInfo B = A.bricks[bid];
-    if (A.done && A.done[B.dom]) return;
     const int tid = threadIdx.x;
     const int* __restrict__ halo = A.halo + (int64_t)bid * NH;
+    // Set-up loads in as few dependent steps as possible: the halo row and the node
+    // masks need only the brick index and are issued before the brick record
+    // arrives; the element list needs the record and is issued before the done
+    // flag is tested (a finished domain's brick wastes these loads)
+    constexpr int NHP = (NH + NT - 1) / NT;
+    int hq[NHP];
+#pragma unroll
+    for (int kk = 0; kk < NHP; ++kk) {
+        const int h = tid + kk * NT;
+        hq[kk] = h < NH ? halo[h] : -1;
+    }
+    const unsigned char nmv = tid < NO ? A.nmask[(int64_t)bid * NO + tid] : (unsigned char)0;
+    // the brick record's fields are all loaded here, in one round trip (the compiler
+    // otherwise loads each field where it is first used, one round trip each)
+    BrickInfo B = A.bricks[bid];
+    asm volatile("" : "+s"(B.dom), "+s"(B.x0), "+s"(B.y0), "+s"(B.z0), "+s"(B.node_begin), "+s"(B.node_end),
+                 "+s"(B.e_off), "+s"(B.e_cnt));
+    static_assert(NET <= (SPARSE ? MV_NT_SP : MV_NT), "one tile element per thread");
+    int eiv = 0;
+    T eEv = T(0);
+    if (tid < B.e_cnt) {
+        eiv = A.einfo[B.e_off + tid];
+        eEv = static_cast<const T*>(A.eE)[B.e_off + tid];
+    }
+    if (A.done && A.done[B.dom]) return;
     const V* xv = reinterpret_cast<const V*>(xin);
     V* yv = reinterpret_cast<V*>(yout);
     const int c0 = (int)blockIdx.y * NCH / (int)gridDim.y, c1 = ((int)blockIdx.y + 1) * NCH / (int)gridDim.y;
@@ -365,12 +388,11 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
     }
     {
         // halo positions h = tid + k NT (one pass with 512 threads, two with 256)
-        constexpr int NHP = (NH + NT - 1) / NT;
         bool exk[NHP];
 #pragma unroll
         for (int kk = 0; kk < NHP; ++kk) {
             const int h = tid + kk * NT;
-            const int q = h < NH ? halo[h] : -1;
+            const int q = hq[kk];
             if (h < NH) s_node[h] = q;
             const bool ex = q >= 0;
             exk[kk] = ex;
@@ -378,7 +400,7 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
             if (ex && hx >= 1 && hx <= BX && hy >= 1 && hy <= BY && hz >= 1 && hz <= BZ)
                 s_ol[(q & NODE_MASK) - B.node_begin] = (unsigned char)((hx - 1) + BX * ((hy - 1) + BY * (hz - 1)));
         }
-        if (tid < NO) s_nm[tid] = A.nmask[(int64_t)bid * NO + tid];
+        if (tid < NO) s_nm[tid] = nmv;
         if constexpr (!EARLY_X) {
             // compact list of the halo positions that hold a node, in position order
             const int lane = tid & 63, wv = tid >> 6;
@@ -404,13 +426,9 @@ __global__ __launch_bounds__(SPARSE ? MV_NT_SP : MV_NT, 4) void k_matvec(MvArgs 
         }
     }
     const int ne = B.e_cnt;
-    {
-        const T* __restrict__ eE = static_cast<const T*>(A.eE) + B.e_off;
-        const int* __restrict__ ei = A.einfo + B.e_off;
-        for (int i = tid; i < ne; i += NT) {
-            s_info[i] = ei[i];
-            s_E[i] = eE[i];
-        }
+    if (tid < ne) {
+        s_info[tid] = eiv;
+        s_E[tid] = eEv;
     }
     __syncthreads();
     if constexpr (SPARSE) {
