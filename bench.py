@@ -34,11 +34,33 @@ for p in (ROOT, PKG):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}   # MI355X dense matrix-core peaks
 # the timed kernels (fine-level launches): kernel_stats name -> kernel
 KERNEL_NAMES = {"mv": "k_matvec<float, IN_V, OUT_MASKED> (Q = K P of the block PCG)",
                 "cheb": "k_matvec<float, IN_V|IN_JS, OUT_CHEB> (Chebyshev-Jacobi smoothing step)",
                 "bres": "k_matvec<float, IN_V, OUT_BRES> (b - K x before restriction)",
                 "resid": "k_matvec<double, IN_U, OUT_RESID> (FP64 true residual)"}
+
+
+def hbm_copy_probe(gib=1.0, iters=10):
+    """Device-to-device copy of a gib-GiB buffer on cuda:0, timed with events: the
+    bandwidth an HBM-bound kernel can reach here (read + write bytes per second, GB/s)."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(2):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    del a, b
+    return 2.0 * n * 4 * iters / (ms * 1e-3) / 1e9
 
 
 def _sync():
@@ -327,10 +349,14 @@ def main():
         if ks["launches"] == 0 or ks["avg_ms"] <= 0:
             continue
         gbs = ks["bytes"] / (ks["avg_ms"] * 1e-3) / 1e9
+        tfl = ks["flops"] / (ks["avg_ms"] * 1e-3) / 1e12
+        pk = MFMA_PEAK_TFLOPS["f64" if name == "resid" else "f32"]
         kern[name] = {"kernel": KERNEL_NAMES[name], "avg_launch_ms": round(ks["avg_ms"], 5),
                       "launches": ks["launches"], "total_ms": round(ks["avg_ms"] * ks["launches"], 2),
                       "algorithmic_bytes": round(ks["bytes"]), "achieved_GBs": round(gbs, 1),
-                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+                      "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "dense_flops": round(ks["flops"]), "dense_TFLOPs": round(tfl, 2),
+                      "flop_frac": round(tfl / pk, 4)}
     run_ms = b.stats()["run_ms"]
     for v in kern.values():
         v["share_of_run"] = round(v["total_ms"] / run_ms, 3)
@@ -373,6 +399,13 @@ def main():
     jac_rate, _, itj = one_by_one(dtc_amd.DTC_FP32_REFINE, dtc_amd.DTC_PC_JACOBI, args.fp64_domains)
     terr = max((np.abs(res[i]["C"] - C64[i]).max() / np.abs(C64[i]).max() for i in range(len(C64))), default=float("nan"))
 
+    # ---- measured copy bandwidth beside the HBM peak
+    try:
+        probe = round(hbm_copy_probe(), 1)
+    except Exception as e:   # reported as missing, never guessed
+        log(f"copy probe failed: {e}")
+        probe = None
+
     # ---- roofline of the dominant kernel (largest share of the run)
     dom_k = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
     roof = None
@@ -388,7 +421,13 @@ def main():
         roof = {"bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": kd["frac"], "traffic": traffic, "kernel": kd["kernel"],
                 "avg_launch_ms": kd["avg_launch_ms"], "launches_timed": kd["launches"],
-                "algorithmic_bytes": kd["algorithmic_bytes"], "share_of_run": kd["share_of_run"]}
+                "algorithmic_bytes": kd["algorithmic_bytes"], "share_of_run": kd["share_of_run"],
+                "copy_probe_GBs": probe,
+                "frac_of_copy_probe": round(kd["achieved_GBs"] / probe, 4) if probe else None,
+                # the flop side: dense element-product count 2 (3 nen)^2 R N_e per launch
+                # against the FP32 matrix-core peak (the Walsh-basis kernel executes fewer)
+                "flop_side": {"achieved": kd["dense_TFLOPs"], "peak": MFMA_PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                              "frac": kd["flop_frac"], "dense_flops": kd["dense_flops"]}}
     for k, v in kern.items():
         tj = os.path.join(ROOT, "profiles", f"traffic_{k}.json")
         if "traffic" not in v and os.path.exists(tj):

@@ -191,6 +191,12 @@ int dtc_batch_timing(dtc_batch* b, int enable);
  * the domains still iterating in that launch.  The FP64 repeat of domains whose
  * mixed solve stalled is not included. */
 int dtc_batch_kernel_stats(dtc_batch* b, int kernel, double* avg_ms, int64_t* launches, double* avg_bytes);
+/* the same launches' average dense element-product flops, 2 (3 nen)^2 R N_e over
+ * the domains still iterating (the flop count of an assembled element-by-element
+ * product; the Walsh-basis kernel executes fewer), for the flop-side roof of the
+ * operator (the counterpart of the per-launch work the reference's
+ * KSPSolve matvec does in f-src/mod_dtc_main_subroutines.f90:995-1194). */
+int dtc_batch_kernel_flops(dtc_batch* b, int kernel, double* avg_flops);
 int dtc_batch_destroy(dtc_batch* b);
 
 /* ---- a12: rotation search ------------------------------------------------

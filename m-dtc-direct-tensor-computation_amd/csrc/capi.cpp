@@ -843,6 +843,20 @@ int dtc_batch_kernel_stats(dtc_batch* b, int kernel, double* avg_ms, int64_t* la
     return DTC_OK;
 }
 
+int dtc_batch_kernel_flops(dtc_batch* b, int kernel, double* avg_flops) {
+    if (!b) return set_error(DTC_ERR_BAD_ARGUMENT, "NULL batch");
+    if (kernel < 0 || kernel >= DTC_KSTAT_COUNT) return set_error(DTC_ERR_BAD_ARGUMENT, "kernel %d", kernel);
+    double fl = 0.0;
+    int64_t n = 0;
+    for (int k = 0; k < b->n_main && k < (int)b->subs.size(); ++k) {
+        const KernelStat& q = b->subs[k]->sb.kstat[kernel];
+        fl += q.flops_sum;
+        n += q.count;
+    }
+    if (avg_flops) *avg_flops = n ? fl / (double)n : 0.0;
+    return DTC_OK;
+}
+
 int dtc_batch_destroy(dtc_batch* b) {
     if (!b) return DTC_OK;
     if (b->st.s) (void)hipStreamSynchronize(b->st.s);

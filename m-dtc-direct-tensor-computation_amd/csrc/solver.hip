@@ -1564,24 +1564,33 @@ __global__ __launch_bounds__(RED_NT) void k_bcheck(const int* __restrict__ dslic
 
 // algorithmic bytes of one timed launch: over the domains not done, vpass
 // vectors (FP32, or FP64 with fp64), the element moduli (4 or 8 bytes each) and,
-// with jac, the FP32 Jacobi scaling; dom_bytes[d] = {V32, E32, J32, V64}.
-// Summed by one workgroup in a fixed order.
+// with jac, the FP32 Jacobi scaling; and its dense element-product flops
+// 2 (3 nen)^2 R N_e; dom_bytes[d] = {V32, E32, J32, V64, F}.  out = {bytes, flops},
+// each summed by one workgroup in a fixed order.
 __global__ __launch_bounds__(256) void k_launch_bytes(const int* __restrict__ done, const double* __restrict__ dom_bytes,
                                                       int n_dom, int vpass, int fp64, int jac, double* __restrict__ out) {
-    __shared__ double sh[256];
-    double acc = 0.0;
+    __shared__ double sh[2][256];
+    double acc = 0.0, fl = 0.0;
     for (int d = threadIdx.x; d < n_dom; d += 256) {
         if (done && done[d]) continue;
-        const double* q = dom_bytes + 4 * d;
+        const double* q = dom_bytes + 5 * d;
         acc += vpass * (fp64 ? q[3] : q[0]) + (fp64 ? 2.0 : 1.0) * q[1] + (jac ? q[2] : 0.0);
+        fl += q[4];
     }
-    sh[threadIdx.x] = acc;
+    sh[0][threadIdx.x] = acc;
+    sh[1][threadIdx.x] = fl;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        if (threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sh[0];
+    if (threadIdx.x == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+    }
 }
 
 __global__ void k_fill_f64(double* __restrict__ a, int64_t n, double v) {
@@ -2911,7 +2920,7 @@ int timed_end(SolverBatch& b, int slot, int vpass, bool fp64, bool jac, const in
     if (slot < 0) return DTC_OK;
     DTC_HIP(hipEventRecord(b.ev_used.back().e1, b.stream));
     k_launch_bytes<<<1, 256, 0, b.stream>>>(done, b.dom_bytes.as<double>(), b.n_dom, vpass, fp64 ? 1 : 0, jac ? 1 : 0,
-                                           b.launch_bytes.as<double>() + slot);
+                                           b.launch_bytes.as<double>() + 2 * slot);
     DTC_HIP(hipGetLastError());
     return DTC_OK;
 }
@@ -2919,14 +2928,15 @@ int timed_end(SolverBatch& b, int slot, int vpass, bool fp64, bool jac, const in
 // after a stream synchronisation: accumulate the timed launches
 int harvest_events(SolverBatch& b) {
     if (b.ev_used.empty()) return DTC_OK;
-    std::vector<double> by((size_t)b.n_slots);
+    std::vector<double> by((size_t)b.n_slots * 2);
     DTC_HIP(hipMemcpy(by.data(), b.launch_bytes.p, by.size() * 8, hipMemcpyDeviceToHost));
     for (const SolverBatch::Timed& t : b.ev_used) {
         float ms = 0.f;
         DTC_HIP(hipEventElapsedTime(&ms, t.e0, t.e1));
         KernelStat& k = b.kstat[t.kind];
         k.ms_sum += ms;
-        k.bytes_sum += by[t.slot];
+        k.bytes_sum += by[2 * t.slot];
+        k.flops_sum += by[2 * t.slot + 1];
         k.count += 1;
         b.ev_free.push_back(t.e0);
         b.ev_free.push_back(t.e1);
@@ -3850,18 +3860,21 @@ int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>
     b.reactions_valid = false;
     b.graph_launches = 0;
     if (b.timing) {
-        // per domain group: vector bytes (FP32), moduli (FP32), Jacobi scaling (FP32), vector bytes (FP64)
-        std::vector<double> db((size_t)D * 4, 0.0);
+        // per domain group: vector bytes (FP32), moduli (FP32), Jacobi scaling (FP32), vector
+        // bytes (FP64), dense element-product flops 2 (3 nen)^2 R N_e
+        std::vector<double> db((size_t)D * 5, 0.0);
+        const double nd = 3.0 * (b.micro20 ? 20 : 8);
         for (int d = 0; d < D; ++d) {
             const double nn = (double)(b.h_dom[d].node_end - b.h_dom[d].node_begin);
             const double ne = b.h_status[d] == DTC_OK ? (double)b.h_elems[d] : 0.0;
-            db[4 * d] = 3.0 * nn * R * 4.0;
-            db[4 * d + 1] = 4.0 * ne;
-            db[4 * d + 2] = 3.0 * nn * 4.0;
-            db[4 * d + 3] = 3.0 * nn * R * 8.0;
+            db[5 * d] = 3.0 * nn * R * 4.0;
+            db[5 * d + 1] = 4.0 * ne;
+            db[5 * d + 2] = 3.0 * nn * 4.0;
+            db[5 * d + 3] = 3.0 * nn * R * 8.0;
+            db[5 * d + 4] = 2.0 * nd * nd * R * ne;
         }
         DTC_TRY(b.dom_bytes.ensure(db.size() * 8));
-        DTC_TRY(b.launch_bytes.ensure((size_t)MAX_SLOTS * 8));
+        DTC_TRY(b.launch_bytes.ensure((size_t)MAX_SLOTS * 2 * 8));
         DTC_HIP(hipMemcpyAsync(b.dom_bytes.p, db.data(), db.size() * 8, hipMemcpyHostToDevice, s));
         DTC_HIP(hipStreamSynchronize(s));
         b.n_slots = 0;

@@ -6,7 +6,7 @@ namespace dtc {
 
 // Accumulated timing of one timed kernel variant.
 struct KernelStat {
-    double ms_sum = 0.0, bytes_sum = 0.0;
+    double ms_sum = 0.0, bytes_sum = 0.0, flops_sum = 0.0;
     int64_t count = 0;
 };
 
@@ -170,8 +170,9 @@ struct SolverBatch {
     std::vector<hipEvent_t> ev, ev_free;
     struct Timed { hipEvent_t e0, e1; int kind; int slot; };
     std::vector<Timed> ev_used;
-    DevMem dom_bytes;              // double [dom][4]: vector bytes V, modulus bytes, Jacobi bytes, FP64 V
-    DevMem launch_bytes;           // double [slot]: algorithmic bytes of a timed launch
+    DevMem dom_bytes;              // double [dom][5]: vector bytes V, modulus bytes, Jacobi bytes, FP64 V,
+                                   // dense element-product flops
+    DevMem launch_bytes;           // double [slot][2]: algorithmic bytes and flops of a timed launch
     int n_slots = 0;
     KernelStat kstat[DTC_KSTAT_COUNT];
     bool timing = false;

@@ -246,12 +246,14 @@ class Batch:
         check("dtc_batch_timing", load().dtc_batch_timing(self._h, int(bool(enable))))
 
     def kernel_stats(self, kernel):
-        """Average launch duration (ms), launches and average algorithmic bytes of one
-        timed kernel (DTC_KSTAT_MV / _CHEB / _BRES / _RESID) over the last run."""
-        avg, n, by = C.c_double(), C.c_int64(), C.c_double()
+        """Average launch duration (ms), launches, average algorithmic bytes and average
+        dense element-product flops of one timed kernel (DTC_KSTAT_MV / _CHEB / _BRES /
+        _RESID) over the last run."""
+        avg, n, by, fl = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
         check("dtc_batch_kernel_stats", load().dtc_batch_kernel_stats(self._h, int(kernel), C.byref(avg), C.byref(n),
                                                                       C.byref(by)))
-        return dict(avg_ms=avg.value, launches=n.value, bytes=by.value)
+        check("dtc_batch_kernel_flops", load().dtc_batch_kernel_flops(self._h, int(kernel), C.byref(fl)))
+        return dict(avg_ms=avg.value, launches=n.value, bytes=by.value, flops=fl.value)
 
     def close(self):
         if self._h:

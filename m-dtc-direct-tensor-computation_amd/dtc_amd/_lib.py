@@ -81,6 +81,7 @@ SIGNATURES = {
     "dtc_batch_stats": (C.c_int, [P, P, P, P, P, P]),
     "dtc_batch_timing": (C.c_int, [P, C.c_int]),
     "dtc_batch_kernel_stats": (C.c_int, [P, C.c_int, P, P, P]),
+    "dtc_batch_kernel_flops": (C.c_int, [P, C.c_int, P]),
     "dtc_batch_destroy": (C.c_int, [P]),
     "dtc_rotation_search": (C.c_int, [P, C.c_int, C.c_double, C.c_double, P, P, P, P, P, P]),
     "dtc_rotation_search_noalloc": (C.c_int, [P, C.c_int, C.c_double, C.c_double, P, P, P, P, P, P]),

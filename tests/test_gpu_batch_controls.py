@@ -100,9 +100,10 @@ def test_precision_used_flag():
 
 def test_kernel_timing_statistics():
     """With timing on, the CG matvec, the fused Chebyshev step, the restriction
-    residual and the FP64 residual are timed; the average algorithmic bytes of a
-    launch lie between one domain's and the whole batch's, for the formula of each
-    kernel.  With timing off nothing is timed."""
+    residual and the FP64 residual are timed; the average algorithmic bytes and the
+    average dense element-product flops of a launch lie between one domain's and the
+    whole batch's, for the formula of each kernel.  With timing off nothing is
+    timed."""
     dtc = _dtc()
     doms = [_binary(dtc, 20, 70 + i) for i in range(3)]
     with dtc.Batch(doms, precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG) as b:
@@ -123,6 +124,9 @@ def test_kernel_timing_statistics():
         s = st[k]
         assert s["launches"] > 0 and s["avg_ms"] > 0.0, (k, s)
         assert lo * (1 - 1e-12) <= s["bytes"] <= hi * (1 + 1e-12), (k, s, lo, hi)
+        # dense element-product flops 2 (3 nen)^2 R N_e of the domains still iterating
+        F = [2 * 24 * 24 * 24 * r["n_elems"] for r in res]
+        assert min(F) * (1 - 1e-12) <= s["flops"] <= sum(F) * (1 + 1e-12), (k, s, F)
     # the block PCG runs one Q = K P per iteration, the V-cycle three fused steps
     assert st[dtc.DTC_KSTAT_CHEB]["launches"] >= 2 * st[dtc.DTC_KSTAT_MV]["launches"]
 
