@@ -43,33 +43,50 @@ KERNEL_NAMES = {"mv": "k_matvec<float, IN_V, OUT_MASKED> (Q = K P of the block P
 
 
 def hbm_copy_probe(gib=1.0, iters=10):
-    """Device-to-device copy of a gib-GiB buffer on cuda:0, timed with events: the
-    bandwidth an HBM-bound kernel can reach here (read + write bytes per second, GB/s)."""
-    import torch
-    n = int(gib * (1 << 30)) // 4
-    a = torch.empty(n, dtype=torch.float32, device="cuda:0")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    for _ in range(2):
-        b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(iters):
-        b.copy_(a)
-    e1.record()
-    e1.synchronize()
-    ms = e0.elapsed_time(e1)
-    del a, b
-    return 2.0 * n * 4 * iters / (ms * 1e-3) / 1e9
+    """Device-to-device copy of a gib-GiB buffer on device 0 through the HIP runtime the
+    library uses (hipMemcpyAsync, timed with HIP events): the bandwidth an HBM-bound
+    kernel can reach here (read + write bytes per second, GB/s)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    n = int(gib * (1 << 30))
+    a, b, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what}: hip error {rc}")
+    ok(hip.hipMalloc(C.byref(a), C.c_size_t(n)), "hipMalloc")
+    try:
+        ok(hip.hipMalloc(C.byref(b), C.c_size_t(n)), "hipMalloc")
+        ok(hip.hipMemset(a, 0, C.c_size_t(n)), "hipMemset")
+        ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        d2d = 3   # hipMemcpyDeviceToDevice
+        for _ in range(2):
+            ok(hip.hipMemcpyAsync(b, a, C.c_size_t(n), d2d, None), "hipMemcpyAsync")
+        ok(hip.hipEventRecord(e0, None), "hipEventRecord")
+        for _ in range(iters):
+            ok(hip.hipMemcpyAsync(b, a, C.c_size_t(n), d2d, None), "hipMemcpyAsync")
+        ok(hip.hipEventRecord(e1, None), "hipEventRecord")
+        ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = C.c_float()
+        ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+        return 2.0 * n * iters / (ms.value * 1e-3) / 1e9
+    finally:
+        for e in (e0, e1):
+            if e.value:
+                hip.hipEventDestroy(e)
+        for p in (a, b):
+            if p.value:
+                hip.hipFree(p)
 
 
 def _sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    """Device synchronisation through the HIP runtime the library uses (the library's
+    own calls return after their stream work; this brackets the timed regions)."""
+    import ctypes as C
+    rc = C.CDLL("libamdhip64.so.7").hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize: hip error {rc}")
 
 
 def make_domains(count, n, seed0):
