@@ -5,7 +5,8 @@
 
 Runs one rocprofv3 --pmc pass (the counters of one pass must fit the hardware's
 per-block limits: at most 8 SQ counters) over one bench.py step of the workload
-(as a child process, the program directly after `--`), and averages every counter
+(as a child process, the program directly after `--`; the step runs as one part,
+DTC_CONCURRENT=1, the form of bench.py's kernel timing run), and averages every counter
 over the fine-level dispatches (the most dispatched grid, the larger on a tie: the
 coarse levels run smaller grids, the launch-shape autotuner a few split ones) of
 each timed kernel: the CG matvec
@@ -44,7 +45,7 @@ def main() -> None:
         cmd = ["rocprofv3", "--pmc"] + counters + ["--kernel-include-regex", "k_matvec", "-d", tmp, "-o", "run",
                                                    "--"] + bench
         print(" ".join(cmd), file=sys.stderr, flush=True)
-        subprocess.run(cmd, check=True, timeout=900, stdout=subprocess.DEVNULL, env=dict(os.environ, TMPDIR="/tmp"))
+        subprocess.run(cmd, check=True, timeout=900, stdout=subprocess.DEVNULL, env=dict(os.environ, TMPDIR="/tmp", DTC_CONCURRENT="1"))
         rows = []
         for db in glob.glob(os.path.join(tmp, "**", "*_results.db"), recursive=True):
             c = sqlite3.connect(db)

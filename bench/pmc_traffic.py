@@ -4,7 +4,8 @@
 
 Runs (as child processes, the program directly after `--`):
   1. rocprofv3 --pmc FETCH_SIZE on one bench.py step of the workload (the
-     multigrid-preconditioned mixed solve, as timed by bench.py),
+     multigrid-preconditioned mixed solve, run as one part, DTC_CONCURRENT=1: the
+     form of bench.py's kernel timing run, whose launches cover the whole batch),
   2. rocprofv3 --pmc WRITE_SIZE on the same run,
   3. both counters on a plain 1 GiB device-to-device copy.
 FETCH_SIZE / WRITE_SIZE are reported in KiB per dispatch.  The copy reads and
@@ -89,7 +90,7 @@ def _pmc(counter: str, out: str, cmd: list[str]) -> str:
             "--"] + cmd
     print(" ".join(full), flush=True)
     p = subprocess.run(full, check=True, timeout=600, stdout=subprocess.PIPE, text=True,
-                       env=dict(os.environ, TMPDIR="/tmp"))
+                       env=dict(os.environ, TMPDIR="/tmp", DTC_CONCURRENT="1"))
     return p.stdout
 
 
