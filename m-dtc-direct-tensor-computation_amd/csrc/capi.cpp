@@ -632,21 +632,27 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         }
         return DTC_OK;
     };
-    // one sub-batch, solve: multigrid hierarchy -> solve -> reactions
-    auto solve_sub = [&](int k) -> int {
+    // one sub-batch, solve: multigrid hierarchy (unless built by prep_sub) -> solve -> reactions
+    auto prep_sub = [&](int k) -> int {
         dtc_batch::Sub& S = *b->subs[k];
         SolverBatch& sb = S.sb;
         if (sb.precond == DTC_PC_MG) {
             if (sb.micro20) DTC_TRY(batch_build_pmg(sb, S.sb08));
             else DTC_TRY(batch_build_mg(sb, 8));
         }
+        return DTC_OK;
+    };
+    auto solve_sub = [&](int k, bool prepared) -> int {
+        dtc_batch::Sub& S = *b->subs[k];
+        SolverBatch& sb = S.sb;
+        if (!prepared) DTC_TRY(prep_sub(k));
         DTC_TRY(batch_solve(sb, b->o, S.iters, S.outer, S.relres));
         DTC_TRY(batch_forces(sb, S.ff));
         return DTC_OK;
     };
     auto run_sub = [&](int k, const std::vector<int>& list, int precision) -> int {
         DTC_TRY(setup_sub(k, list, precision));
-        return solve_sub(k);
+        return solve_sub(k, false);
     };
     for (int k = 0; k < NS; ++k) {
         std::vector<int> list;
@@ -654,10 +660,17 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         DTC_TRY(setup_sub(k, list, b->o.precision));
     }
     if (!concurrent) {
-        for (int k = 0; k < NS; ++k) DTC_TRY(solve_sub(k));
+        for (int k = 0; k < NS; ++k) DTC_TRY(solve_sub(k, false));
     } else {
-        // concurrent solves (one host thread per sub-batch, each on its stream);
+        // multigrid hierarchies and launch-shape tuning one sub-batch after another
+        // (the tuning launches are timed: none may overlap another sub-batch's work),
+        // then concurrent solves (one host thread per sub-batch, each on its stream);
         // the error message of a failed solve is carried back to this thread
+        for (int k = 0; k < NS; ++k) {
+            DTC_TRY(prep_sub(k));
+            DTC_TRY(batch_autotune(b->subs[k]->sb));
+            DTC_HIP(hipStreamSynchronize(b->subs[k]->sb.stream));
+        }
         std::vector<int> rc(NS, DTC_OK);
         std::vector<std::string> msg(NS);
         std::vector<std::thread> th;
@@ -668,7 +681,7 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
                     msg[k] = "hipSetDevice failed in a solve thread";
                     return;
                 }
-                rc[k] = solve_sub(k);
+                rc[k] = solve_sub(k, true);
                 if (rc[k] != DTC_OK) msg[k] = dtc_last_error();
             });
         for (std::thread& t : th) t.join();

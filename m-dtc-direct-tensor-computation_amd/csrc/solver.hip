@@ -3806,6 +3806,15 @@ int autotune_splits(SolverBatch& b, const WK<float>& wk) {
 
 int batch_build_pmg(SolverBatch& b, SolverBatch& c) { return build_pmg(b, c); }
 
+// the launch-shape tuning of a mixed multigrid solve, ahead of it (after the
+// multigrid hierarchy is built): sub-batches solved at the same time are tuned one
+// after another before their solves start, so that no tuning launch overlaps
+// another sub-batch's kernels (batch_solve then finds the choices cached)
+int batch_autotune(SolverBatch& b) {
+    if (b.precision != DTC_FP32_REFINE || b.precond != DTC_PC_MG || (b.micro20 && !b.p08)) return DTC_OK;
+    return autotune_splits(b, cast_wk<float>(b.wk64));
+}
+
 int batch_solve(SolverBatch& b, const dtc_solve_opts& opts, std::vector<int64_t>& iters,
                 std::vector<int64_t>& outer, std::vector<double>& relres) {
     const int D = b.n_dom;

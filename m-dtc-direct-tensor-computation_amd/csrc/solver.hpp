@@ -211,6 +211,7 @@ int batch_build_mg(SolverBatch& b, int max_levels);
 // p-multigrid of a HEX20 micro batch b over c, the HEX08 discretisation of the same
 // voxels and domain groups (both finalised): builds c's levels and links c to b.
 int batch_build_pmg(SolverBatch& b, SolverBatch& c);
+int batch_autotune(SolverBatch& b);
 // Full solve: returns per-domain status in b.h_status; fills iteration counts.
 int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& iters,
                 std::vector<int64_t>& outer, std::vector<double>& relres);
