@@ -515,6 +515,7 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
     b->ng.assign(D, 0);
     std::vector<int> first(1, 0);
     bool concurrent = false;
+    int conc = 1;
     {
         int64_t cap = NODE_CAP;   // DTC_SUBBATCH_NODES lowers it (smaller sub-batches)
         if (const char* e = std::getenv("DTC_SUBBATCH_NODES")) cap = std::max<int64_t>(1, std::min<int64_t>(cap, std::atoll(e)));
@@ -539,10 +540,13 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         // 3 parts 20.5-20.8, 4 parts 19.0 domains/s.  With kernel timing on, one part
         // (the kernels of the parts would overlap, and per-kernel timings would no
         // longer measure one kernel alone).  DTC_CONCURRENT=k sets the parts.
+        // A batch of several sub-batches solves them CONC_DEFAULT at a time (all are
+        // set up, and hold their memory, before the first solve starts).
         constexpr int CONC_DEFAULT = 3;
-        int conc = b->timing ? 1 : CONC_DEFAULT;
+        conc = b->timing ? 1 : CONC_DEFAULT;
         if (const char* e = std::getenv("DTC_CONCURRENT")) conc = std::max(1, std::atoi(e));
         conc = std::min(conc, D);
+        if (first.size() > 2 && conc > 1) concurrent = true;
         if (first.size() == 2 && conc > 1) {
             int64_t total = 0;
             std::vector<int64_t> est(D);
@@ -664,8 +668,8 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
     } else {
         // multigrid hierarchies and launch-shape tuning one sub-batch after another
         // (the tuning launches are timed: none may overlap another sub-batch's work),
-        // then concurrent solves (one host thread per sub-batch, each on its stream);
-        // the error message of a failed solve is carried back to this thread
+        // then concurrent solves, conc at a time (one host thread per sub-batch, each
+        // on its stream); the error message of a failed solve is carried back to this thread
         for (int k = 0; k < NS; ++k) {
             DTC_TRY(prep_sub(k));
             DTC_TRY(batch_autotune(b->subs[k]->sb));
@@ -673,18 +677,21 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         }
         std::vector<int> rc(NS, DTC_OK);
         std::vector<std::string> msg(NS);
-        std::vector<std::thread> th;
-        for (int k = 0; k < NS; ++k)
-            th.emplace_back([&, k] {
-                if (hipSetDevice(device) != hipSuccess) {
-                    rc[k] = DTC_ERR_DEVICE;
-                    msg[k] = "hipSetDevice failed in a solve thread";
-                    return;
-                }
-                rc[k] = solve_sub(k, true);
-                if (rc[k] != DTC_OK) msg[k] = dtc_last_error();
-            });
-        for (std::thread& t : th) t.join();
+        const int wave = std::max(conc, 2);
+        for (int k0 = 0; k0 < NS; k0 += wave) {
+            std::vector<std::thread> th;
+            for (int k = k0; k < std::min(NS, k0 + wave); ++k)
+                th.emplace_back([&, k] {
+                    if (hipSetDevice(device) != hipSuccess) {
+                        rc[k] = DTC_ERR_DEVICE;
+                        msg[k] = "hipSetDevice failed in a solve thread";
+                        return;
+                    }
+                    rc[k] = solve_sub(k, true);
+                    if (rc[k] != DTC_OK) msg[k] = dtc_last_error();
+                });
+            for (std::thread& t : th) t.join();
+        }
         for (int k = 0; k < NS; ++k)
             if (rc[k] != DTC_OK) return set_error(rc[k], "%s", msg[k].c_str());
     }
