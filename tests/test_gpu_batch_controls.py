@@ -264,3 +264,23 @@ def test_fused_zero_start_smoothing_equals_the_stored_form(monkeypatch):
             for k in ("C", "K_macro", "ff"):
                 assert np.array_equal(a[k], c[k]), k
 
+
+
+def test_concurrent_sub_batches_equal_sequential(monkeypatch):
+    """A batch that the solver-node cap splits into several sub-batches solves them
+    several at a time (the default), each on its own stream after all are set up and
+    tuned one after another; every sub-batch's arithmetic is that of the sequential
+    run (the launch shapes only regroup work), so the results agree bit for bit."""
+    dtc = _dtc()
+    doms = [_binary(dtc, 16 + 2 * i, 110 + i) for i in range(5)] + [_grey(dtc, 18, 115)]
+    kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+    monkeypatch.setenv("DTC_SUBBATCH_NODES", "9000")
+    out = []
+    for conc in ("1", "3"):
+        monkeypatch.setenv("DTC_CONCURRENT", conc)
+        out.append(dtc.solve_batch(doms, **kw))
+    for a, c in zip(*out):
+        assert a["status"] == c["status"] == 0
+        assert a["iterations"] == c["iterations"] and a["outer_iterations"] == c["outer_iterations"]
+        for k in ("C", "K_macro", "ff"):
+            assert np.array_equal(a[k], c[k]), k
