@@ -663,6 +663,8 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
         for (int i = first[k]; i < first[k + 1]; ++i) list.push_back(i);
         DTC_TRY(setup_sub(k, list, b->o.precision));
     }
+    DTC_HIP(hipDeviceSynchronize());
+    const auto t_setup = std::chrono::steady_clock::now();
     if (!concurrent) {
         for (int k = 0; k < NS; ++k) DTC_TRY(solve_sub(k, false));
     } else {
@@ -773,6 +775,13 @@ int dtc_batch_run(dtc_batch* b, dtc_domain_result* res) {
     }
     DTC_HIP(hipStreamSynchronize(s));
     b->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    // DTC_TRACE=1: where the run's time went (set-up = meshing, tables and the batch
+    // data of every sub-batch; the rest = multigrid, solves, reactions, fit, rotation)
+    if (const char* te = std::getenv("DTC_TRACE"); te && te[0] == '1')
+        std::fprintf(stderr, "[dtc] run: %d sub-batch(es)%s, set-up %.1f ms, solve and results %.1f ms\n", NS,
+                     concurrent ? " at the same time" : "",
+                     std::chrono::duration<double, std::milli>(t_setup - t0).count(),
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_setup).count());
     std::memcpy(res, out.data(), sizeof(dtc_domain_result) * D);
     b->ran = true;
     return DTC_OK;
