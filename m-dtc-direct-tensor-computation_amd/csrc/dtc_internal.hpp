@@ -143,6 +143,13 @@ struct BrickInfo {
     int e_off, e_cnt;         // solid elements of the element tile in the brick element tables
 };
 
+// New Chebyshev-Jacobi direction of one unknown, c1 d + c2 D^-1 (b - t) with t = (K x)
+// of the unknown: one expression for every kernel that forms it, so that a step fused
+// into an operator's epilogue rounds like the separate vector kernel
+__device__ __forceinline__ float cheb_dir(float c1, float d, float c2, float di, float b, float t) {
+    return c1 * d + c2 * di * (b - t);
+}
+
 // brick halo table entries: solver node | BC_BIT for a Dirichlet node, -1 = no node
 constexpr int BC_BIT = 1 << 30;
 constexpr int NODE_MASK = BC_BIT - 1;

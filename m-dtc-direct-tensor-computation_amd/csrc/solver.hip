@@ -2199,7 +2199,7 @@ __global__ __launch_bounds__(NT_UPD) void k_cheb(const BrickInfo* __restrict__ b
             dv = c2 * di * bv;
             xv = dv;
         } else {
-            dv = c1 * d[idx] + c2 * di * (bv - t[idx]);
+            dv = cheb_dir(c1, d[idx], c2, di, bv, t[idx]);
             xv = x[idx] + dv;
         }
         if (d) d[idx] = dv;
@@ -3260,6 +3260,9 @@ int pmg_vcycle(SolverBatch& b, const float* r, float* z, double* prz) {
             double* dk = k == deg - 1 ? dot : nullptr;
             if (zero && k == 0) {
                 k_cheb<<<b.n_bricks, NT_UPD, 0, s>>>(bl, done, r, nullptr, b.dinv32.as<float>(), d, z, c1, c2, 1, dk);
+            } else if (!dk && cheb_fuse() && !hex20_gather_form()) {
+                // the step in the epilogue of the operator's node gather (no t vector)
+                DTC_TRY(mv20_cheb(b, z, done, r, b.dinv32.as<float>(), d, c1, c2));
             } else {
                 DTC_TRY(mv20(b, false, IN_V, OUT_MASKED, z, t, nullptr, done));
                 k_cheb<<<b.n_bricks, NT_UPD, 0, s>>>(bl, done, r, t, b.dinv32.as<float>(), d, z, c1, c2, 0, dk);

@@ -212,6 +212,16 @@ int batch_build_mg(SolverBatch& b, int max_levels);
 // voxels and domain groups (both finalised): builds c's levels and links c to b.
 int batch_build_pmg(SolverBatch& b, SolverBatch& c);
 int batch_autotune(SolverBatch& b);
+// DTC_HEX20_GATHER=1: the HEX20 operator as the node-gather kernel (the reference form;
+// no Chebyshev epilogue), else element pass + node sums
+inline bool hex20_gather_form() {
+    const char* e = std::getenv("DTC_HEX20_GATHER");
+    return e && e[0] == '1';
+}
+// HEX20 FP32 operator with one Chebyshev-Jacobi step in its epilogue: on free rows
+// d = c1 d + c2 dinv (bv - K x), x += d (x updated in place; Dirichlet rows: K x = 0)
+int mv20_cheb(SolverBatch& b, float* x, const int* done, const float* bv, const float* dinv, float* d, float c1,
+              float c2);
 // Full solve: returns per-domain status in b.h_status; fills iteration counts.
 int batch_solve(SolverBatch& b, const dtc_solve_opts& o, std::vector<int64_t>& iters,
                 std::vector<int64_t>& outer, std::vector<double>& relres);

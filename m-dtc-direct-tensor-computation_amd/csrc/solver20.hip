@@ -403,8 +403,18 @@ __global__ __launch_bounds__(256) void k20_elem_mf(Dom20Dev G, int dom, const in
     }
 }
 
+// Operands of the Chebyshev epilogue of k20_gather (OUT 4)
+struct Cheb20 {
+    const float* b;      // right-hand side
+    const float* dinv;   // Jacobi scaling per (node, direction)
+    float* d;            // direction, read and written
+    float c1, c2;
+};
+
 // y on the pseudo-bricks: the sum of the adjacent elements' forces; MASKED zeroes
-// Dirichlet rows, MASKED_DOT also writes per-brick sums of x . y
+// Dirichlet rows, MASKED_DOT also writes per-brick sums of x . y; OUT 4 (FP32) applies
+// one Chebyshev-Jacobi step to x in place instead of writing y: with t the masked sum,
+// d = c1 d + c2 dinv (b - t), x = x + d (yout is x)
 // (T: the type of the element forces and of the sums; x is an FP32 vector, y one
 // of type TO)
 template <typename T, int OUT, typename TO = float>
@@ -412,7 +422,9 @@ __global__ __launch_bounds__(NT20) void k20_gather(const BrickInfo* __restrict__
                                                   const Dom20Dev* __restrict__ d20,
                                                   const int64_t* __restrict__ ebase, const int* __restrict__ done,
                                                   const T* __restrict__ ye, const float* __restrict__ xin,
-                                                  TO* __restrict__ yout, double* __restrict__ partial) {
+                                                  TO* __restrict__ yout, double* __restrict__ partial,
+                                                  Cheb20 ch) {
+    static_assert(OUT != 4 || (sizeof(T) == 4 && sizeof(TO) == 4), "Chebyshev epilogue: FP32");
     __shared__ double s_red[NT20];
     const BrickInfo B = bricks[blockIdx.x];
     if (done && done[B.dom]) return;
@@ -452,8 +464,14 @@ __global__ __launch_bounds__(NT20) void k20_gather(const BrickInfo* __restrict__
         for (int d = 0; d < 3; ++d) {
             const int64_t o = ((int64_t)q * 3 + d) * R + r;
             const TO v = bc ? TO(0) : (TO)y[d];
-            if (OUT == 0) acc += (double)xin[o] * (double)v;
-            yout[o] = v;
+            if constexpr (OUT == 4) {
+                const float dv = cheb_dir(ch.c1, ch.d[o], ch.c2, ch.dinv[o / R], ch.b[o], (float)v);
+                ch.d[o] = dv;
+                yout[o] = (TO)(yout[o] + dv);
+            } else {
+                if (OUT == 0) acc += (double)xin[o] * (double)v;
+                yout[o] = v;
+            }
         }
     }
     if (OUT == 0) {
@@ -481,41 +499,59 @@ int mv20_launch(SolverBatch& b, const void* xin, void* yout, double* part, const
 
 }  // namespace
 
+// FP32 element forces of every domain group into ye20 (on the matrix cores;
+// DTC_HEX20_MFMA=0: the VALU form)
+int elem_pass32(SolverBatch& b, const float* xin, const int* done) {
+    const char* me = std::getenv("DTC_HEX20_MFMA");
+    const bool use_mf = !(me && me[0] == '0');
+    hipStream_t s = b.stream;
+    for (int g = 0; g < b.n_dom; ++g) {
+        const Dom20Dev& G = b.h_d20dev[g];
+        const int64_t ne = b.h_elems[g];   // the count ye20 was sized for (0 for a domain that failed set-up)
+        if (ne <= 0 || !G.elems) continue;
+        float* ye = b.ye20.as<float>() + b.h_ebase20[g] * 60 * R;
+        if (use_mf)
+            k20_elem_mf<float><<<nb(ne, EmCfg<float>::EM), 256, 0, s>>>(G, g, done, b.k20_32.as<float>(), xin, ye, ne);
+        else
+            k20_elem<<<nb(ne, E20_EPB), E20_NT, 0, s>>>(G, g, done, b.k20_32.as<float>(), xin, ye, ne);
+    }
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
+int mv20_cheb(SolverBatch& b, float* x, const int* done, const float* bv, const float* dinv, float* d, float c1,
+              float c2) {
+    if (hex20_gather_form()) return set_error(DTC_ERR_UNSUPPORTED, "Chebyshev epilogue of the node-gather form");
+    if (b.n_bricks == 0) return DTC_OK;
+    DTC_TRY(elem_pass32(b, x, done));
+    k20_gather<float, 4><<<b.n_bricks, NT20, 0, b.stream>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
+                                                         b.ebase20.as<int64_t>(), done, b.ye20.as<float>(), nullptr, x,
+                                                         nullptr, Cheb20{bv, dinv, d, c1, c2});
+    DTC_HIP(hipGetLastError());
+    return DTC_OK;
+}
+
 int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout, double* part, const int* done) {
     if (fp64) {
         if (in == 0 && out == 0) return mv20_launch<double, 0, 0>(b, xin, yout, part, done);
         if (in == 1 && out == 1) return mv20_launch<double, 1, 1>(b, xin, yout, part, done);
         if (in == 1 && out == 2) return mv20_launch<double, 1, 2>(b, xin, yout, part, done);
     } else if (in == 0 && (out == 0 || out == 3)) {
-        const char* ee = std::getenv("DTC_HEX20_GATHER");
-        if (ee && ee[0] == '1') {   // the node-gather kernel (reference form of the operator)
+        if (hex20_gather_form()) {   // the node-gather kernel (reference form of the operator)
             if (out == 0) return mv20_launch<float, 0, 0>(b, xin, yout, part, done);
             return mv20_launch<float, 0, 3>(b, xin, yout, part, done);
         }
         if (b.n_bricks == 0) return DTC_OK;
-        // element products on the matrix cores (DTC_HEX20_MFMA=0: the VALU form)
-        const char* me = std::getenv("DTC_HEX20_MFMA");
-        const bool use_mf = !(me && me[0] == '0');
         hipStream_t s = b.stream;
-        for (int g = 0; g < b.n_dom; ++g) {
-            const Dom20Dev& G = b.h_d20dev[g];
-            const int64_t ne = b.h_elems[g];   // the count ye20 was sized for (0 for a domain that failed set-up)
-            if (ne <= 0 || !G.elems) continue;
-            float* ye = b.ye20.as<float>() + b.h_ebase20[g] * 60 * R;
-            if (use_mf)
-                k20_elem_mf<float><<<nb(ne, EmCfg<float>::EM), 256, 0, s>>>(G, g, done, b.k20_32.as<float>(),
-                                                                            (const float*)xin, ye, ne);
-            else
-                k20_elem<<<nb(ne, E20_EPB), E20_NT, 0, s>>>(G, g, done, b.k20_32.as<float>(), (const float*)xin, ye, ne);
-        }
+        DTC_TRY(elem_pass32(b, (const float*)xin, done));
         if (out == 0)
             k20_gather<float, 0><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
                                                       b.ebase20.as<int64_t>(), done, b.ye20.as<float>(),
-                                                      (const float*)xin, (float*)yout, part);
+                                                      (const float*)xin, (float*)yout, part, Cheb20{});
         else
             k20_gather<float, 3><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
                                                       b.ebase20.as<int64_t>(), done, b.ye20.as<float>(),
-                                                      (const float*)xin, (float*)yout, part);
+                                                      (const float*)xin, (float*)yout, part, Cheb20{});
         DTC_HIP(hipGetLastError());
         return DTC_OK;
     }
@@ -525,8 +561,7 @@ int mv20(SolverBatch& b, bool fp64, int in, int out, const void* xin, void* yout
 int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done, double* q64) {
     if (b.n_bricks == 0) return DTC_OK;
     hipStream_t s = b.stream;
-    const char* ee = std::getenv("DTC_HEX20_GATHER");
-    if (ee && ee[0] == '1') {   // the node-gather form
+    if (hex20_gather_form()) {   // the node-gather form
         Mv20 a{b.brick.as<BrickInfo>(), b.dom.as<DomInfo>(), b.d20dev.as<Dom20Dev>(), done, nullptr};
         if (q64) return set_error(DTC_ERR_UNSUPPORTED, "HEX20 node-gather operator with an FP64 output");
         k_matvec20<double, 0, 3, float><<<b.n_bricks, NT20, 0, s>>>(a, b.k20_64.as<double>(), p, q);
@@ -543,11 +578,11 @@ int mv20_q64(SolverBatch& b, const float* p, float* q, const int* done, double* 
     if (q64)
         k20_gather<double, 3, double><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
                                                                   b.ebase20.as<int64_t>(), done, b.ye20.as<double>(),
-                                                                  p, q64, nullptr);
+                                                                  p, q64, nullptr, Cheb20{});
     else
         k20_gather<double, 3><<<b.n_bricks, NT20, 0, s>>>(b.brick.as<BrickInfo>(), b.d20dev.as<Dom20Dev>(),
                                                           b.ebase20.as<int64_t>(), done, b.ye20.as<double>(), p, q,
-                                                          nullptr);
+                                                          nullptr, Cheb20{});
     DTC_HIP(hipGetLastError());
     return DTC_OK;
 }
