@@ -170,7 +170,8 @@ def uneven(args, log):
     doms = [dtc_amd.Domain(p, llimit=1, delta=(0.01, 0.01, 0.01), xa_n=(2, 2, 2)) for p in phis]
     res, dt, G = [], 0.0, 128      # runs of 128 domains (each split into sub-batches by the library)
     for g0 in range(0, len(doms), G):
-        with dtc_amd.Batch(doms[g0:g0 + G], precision=dtc_amd.DTC_FP32_REFINE, rel_tol=1e-8, rotation=True,
+        with dtc_amd.Batch(doms[g0:g0 + G], precision=dtc_amd.DTC_FP32_REFINE, rel_tol=1e-8,
+                           inner_tol=args.inner_tol, rotation=True,
                            rot_step_deg=5.0, precond=dtc_amd.DTC_PC_MG, max_iter=args.max_iter) as b:
             _sync()
             t0 = time.perf_counter()
