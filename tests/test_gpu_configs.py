@@ -55,7 +55,8 @@ def test_config2_trabecular_64_vs_oracle_fixture():
     """The metric's workload domain (64^3, seed 2, BV/TV 0.20, sigma 2): the plain FP64
     and the mixed (FP32 block PCG + multigrid, FP64 refinement) GPU solves against the
     oracle's solution at rel-res 1e-10 (tests/golden/trabecular64_seed2.npz, written by
-    tests/golden/make_trabecular64.py): within 1e-6 (FP64) and 1e-5 (mixed)."""
+    tests/golden/make_trabecular64.py): within 1e-6 (FP64) and 1e-5 (mixed), the mixed
+    solve also as one domain of a three-domain batch (concurrent parts)."""
     dtc = _dtc()
     import os
     import sys
@@ -76,6 +77,14 @@ def test_config2_trabecular_64_vs_oracle_fixture():
     assert r32["precision_used"] == dtc.DTC_FP32_REFINE
     assert _rel(r32["C"], r64["C"]) < 1e-5 and _rel(r32["K_macro"], r64["K_macro"]) < 1e-5
     assert r32["iterations"] < r64["iterations"] / 4          # multigrid vs Jacobi
+    # the same domain inside a batch (three domains, solved as concurrent parts by default)
+    from dtc_amd import synth
+    others = [dtc.Domain(synth.trabecular(64, seed=sd, sigma=2.0, bvtv=0.20), llimit=1, delta=(G.DELTA,) * 3,
+                         xa_n=G.XA, E=G.E, nu=G.NU) for sd in (3, 4)]
+    rb = dtc.solve_batch([others[0], d, others[1]], precision=1, rel_tol=1e-8, rotation=True, precond=1)
+    assert all(x["status"] == 0 and x["relres"] <= 1e-8 for x in rb)
+    for key, ref in (("C", "C"), ("EE", "EE"), ("ff", "ff"), ("K_macro", "K_M")):
+        assert _rel(rb[1][key], fx[ref]) < 1e-5, ("batch", key, _rel(rb[1][key], fx[ref]))
 
 
 def test_config3_uneven_batch_edges_32_to_96():
