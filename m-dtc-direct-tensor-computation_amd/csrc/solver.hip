@@ -1498,17 +1498,29 @@ __global__ __launch_bounds__(NT_UPD) void k_bupdate64(const int4* __restrict__ s
 // pn = Z + P coef (z null: pn = P coef); pn aliases neither p nor z.  With beta
 // (k_bsolve mode 1) the new direction block, with L^-T (mode 2) its orthonormal
 // columns
+// GRAM: also the per-slice Gram partials pnᵀpn of the rows written (for the
+// orthonormalisation that follows): the block's new rows are staged in LDS and wave w
+// sums rows w, w + 3, ... of each block, lane (i, j) its 3 x 3 entries, in FP64 (the
+// layout of k_gram_lds's partials, another summation order)
+template <bool GRAM>
 __global__ __launch_bounds__(NT_UPD) void k_bdir(const int4* __restrict__ slices, const int* __restrict__ done,
                                                 const float* __restrict__ coef, const float* __restrict__ p,
-                                                const float* __restrict__ z, float* __restrict__ pn) {
+                                                const float* __restrict__ z, float* __restrict__ pn,
+                                                double* __restrict__ gpart) {
     __shared__ float4 s_c[R * 6];
     __shared__ float4 s_p[NT_UPD / 6][6];   // P rows of the current block (staged as in k_bupdate)
+    __shared__ float4 s_n[GRAM ? NT_UPD / 6 : 1][6];   // GRAM: the block's new rows
+    __shared__ double s_g[GRAM ? RR : 1];
     const int4 sl = slices[blockIdx.x];   // domain, first node, end node
     if (done[sl.x]) return;
     const int tid = threadIdx.x, qd = tid % 6, rw = tid / 6;
+    const int wave = tid >> 6, lane = tid & 63, ci = 3 * (lane >> 3), cj = 3 * (lane & 7);
     const float4* c4 = reinterpret_cast<const float4*>(coef + (int64_t)sl.x * RR);
     for (int e = tid; e < R * 6; e += NT_UPD) s_c[e] = c4[e];
-    constexpr int RB = NT_UPD / 6;
+    constexpr int RB = NT_UPD / 6, NWV = NT_UPD / 64;
+    double g[GRAM ? 9 : 1];
+#pragma unroll
+    for (int k = 0; k < (GRAM ? 9 : 1); ++k) g[k] = 0.0;
     const int64_t r0 = (int64_t)sl.y * 3, r1 = (int64_t)sl.z * 3;
     float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
     auto load = [&](int64_t b0) {
@@ -1522,19 +1534,54 @@ __global__ __launch_bounds__(NT_UPD) void k_bdir(const int4* __restrict__ slices
         __syncthreads();
         if (b0 + RB < r1) load(b0 + RB);
         const int64_t row = b0 + rw;
-        if (row >= r1) continue;
-        float4 v = z ? reinterpret_cast<const float4*>(z + row * R)[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < r1) {
+            if (z) v = reinterpret_cast<const float4*>(z + row * R)[qd];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const float4 pv = s_p[rw][c];
+            for (int c = 0; c < 6; ++c) {
+                const float4 pv = s_p[rw][c];
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const float4 a = s_c[(4 * c + w) * 6 + qd];
-                const float pk = vcomp(pv, w);
-                v.x += pk * a.x; v.y += pk * a.y; v.z += pk * a.z; v.w += pk * a.w;
+                for (int w = 0; w < 4; ++w) {
+                    const float4 a = s_c[(4 * c + w) * 6 + qd];
+                    const float pk = vcomp(pv, w);
+                    v.x += pk * a.x; v.y += pk * a.y; v.z += pk * a.z; v.w += pk * a.w;
+                }
+            }
+            reinterpret_cast<float4*>(pn + row * R)[qd] = v;
+        }
+        if constexpr (GRAM) {
+            s_n[rw][qd] = v;   // rows past the slice hold zeros
+            __syncthreads();
+            const float* sn = reinterpret_cast<const float*>(s_n);
+            for (int rr = wave; rr < RB; rr += NWV) {
+                double ad[3], bd[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    ad[i] = (double)sn[rr * R + ci + i];
+                    bd[i] = (double)sn[rr * R + cj + i];
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) g[3 * i + j] = fma(ad[i], bd[j], g[3 * i + j]);
             }
         }
-        reinterpret_cast<float4*>(pn + row * R)[qd] = v;
+    }
+    if constexpr (GRAM) {
+        __syncthreads();
+        for (int w = 0; w < NWV; ++w) {
+            if (wave == w)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const int e = (ci + i) * R + cj + j;
+                        s_g[e] = w == 0 ? g[3 * i + j] : s_g[e] + g[3 * i + j];
+                    }
+            __syncthreads();
+        }
+        double* out = gpart + (int64_t)blockIdx.x * 2 * RR;
+        for (int e = tid; e < 2 * RR; e += NT_UPD) out[e] = e < RR ? s_g[e] : 0.0;
     }
 }
 
@@ -3420,6 +3467,13 @@ int block_setup(SolverBatch& b) {
 
 // per-domain Grams a^T b and a^T c (c may be null) into gsum
 template <typename T> struct NoDeduce { using type = T; };
+// the Gram partials of the orthonormalisation formed in the direction kernel that
+// writes the new block (DTC_ORTH_FUSE=0: a separate Gram kernel reads it back)
+inline bool orth_fuse() {
+    const char* e = std::getenv("DTC_ORTH_FUSE");
+    return !(e && e[0] == '0');
+}
+
 // Gram form: rows staged through LDS (default) or read by every lane from global
 // memory (DTC_GRAM_LDS=0; the same sums, for measurements)
 inline bool gram_lds() {
@@ -3451,14 +3505,35 @@ int block_gram(SolverBatch& b, const TA* a, const TB* bb, const typename NoDeduc
 // FP32, combinations of nearly dependent columns with large coefficients that
 // cancel lose the directions they should keep: the residual columns of a
 // domain line up with its slowest modes after the first residual replacement.
-int block_orth(SolverBatch& b, const float* src, float* dst) {
+int block_orth(SolverBatch& b, const float* src, float* dst, bool partials_done = false) {
     hipStream_t s = b.stream;
     int* done = b.done.as<int>();
-    DTC_TRY(block_gram(b, src, src, nullptr));
+    if (partials_done) {   // the direction kernel wrote src's Gram partials
+        k_gram_sum<<<dim3(b.n_dom, 2 * RR / 64), 256, 0, s>>>(b.dslice.as<int>(), done, b.gpart.as<double>(),
+                                                             b.gsum.as<double>());
+    } else {
+        DTC_TRY(block_gram(b, src, src, nullptr));
+    }
     k_bsolve<<<b.n_dom, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 2, b.lfac.as<double>(), b.bcoef.as<float>());
-    k_bdir<<<b.n_slices, NT_UPD, 0, s>>>(b.slices.as<int4>(), done, b.bcoef.as<float>(), src, nullptr, dst);
+    k_bdir<false><<<b.n_slices, NT_UPD, 0, s>>>(b.slices.as<int4>(), done, b.bcoef.as<float>(), src, nullptr, dst,
+                                                nullptr);
     DTC_HIP(hipGetLastError());
     return DTC_OK;
+}
+
+// the new direction block Z + P beta and, with orth_fuse(), its Gram partials in the
+// same kernel; then the orthonormalisation into p
+int block_dir_orth(SolverBatch& b, float* p, const float* z, float* q) {
+    hipStream_t s = b.stream;
+    int* done = b.done.as<int>();
+    const int4* sl = b.slices.as<int4>();
+    const bool fuse = orth_fuse() && b.n_slices > 0;
+    if (fuse)
+        k_bdir<true><<<b.n_slices, NT_UPD, 0, s>>>(sl, done, b.bcoef.as<float>(), p, z, q, b.gpart.as<double>());
+    else
+        k_bdir<false><<<b.n_slices, NT_UPD, 0, s>>>(sl, done, b.bcoef.as<float>(), p, z, q, nullptr);
+    DTC_HIP(hipGetLastError());
+    return block_orth(b, q, p, fuse);
 }
 
 // one block PCG iteration on FP64 vectors (x64, r64) with the FP32 multigrid as
@@ -3488,9 +3563,7 @@ int block_iteration64(SolverBatch& b, const WK<float>& wk, float* p, float* q, f
     DTC_TRY(mg_vcycle(b, wk, b.r32.as<float>(), z, nullptr));
     DTC_TRY((block_gram<double, float>(b, q64, z, nullptr)));
     k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 1, b.lfac.as<double>(), b.bcoef.as<float>());
-    k_bdir<<<NSl, NT_UPD, 0, s>>>(sl, done, b.bcoef.as<float>(), p, z, q);
-    DTC_HIP(hipGetLastError());
-    return block_orth(b, q, p);
+    return block_dir_orth(b, p, z, q);
 }
 
 // one block PCG iteration on (x, r); p is the direction block (orthonormal
@@ -3514,9 +3587,7 @@ int block_iteration(SolverBatch& b, const WK<float>& wk, float* x, float* r, flo
     DTC_TRY(mg_vcycle(b, wk, r, z, nullptr));
     DTC_TRY(block_gram(b, q, z, nullptr));
     k_bsolve<<<D, BS_NT, 0, s>>>(done, b.gsum.as<double>(), 1, b.lfac.as<double>(), b.bcoef.as<float>());
-    k_bdir<<<NSl, NT_UPD, 0, s>>>(sl, done, b.bcoef.as<float>(), p, z, q);
-    DTC_HIP(hipGetLastError());
-    return block_orth(b, q, p);
+    return block_dir_orth(b, p, z, q);
 }
 
 // *out = 1 when every domain's flag is set (one workgroup, fixed order)

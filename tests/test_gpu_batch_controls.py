@@ -284,3 +284,23 @@ def test_concurrent_sub_batches_equal_sequential(monkeypatch):
         assert a["iterations"] == c["iterations"] and a["outer_iterations"] == c["outer_iterations"]
         for k in ("C", "K_macro", "ff"):
             assert np.array_equal(a[k], c[k]), k
+
+
+def test_fused_orthonormalisation_gram_agrees_with_the_separate_kernel(monkeypatch):
+    """The Gram of the new direction block formed in the direction kernel (default)
+    adds the same products as the separate Gram kernel (DTC_ORTH_FUSE=0) in another
+    order, so the two solves are two roundings of one iteration: the tensors agree
+    to what rel-res 1e-8 fixes and the iteration counts within 2 (binary batch on FP32
+    vectors, greyscale batch on FP64 vectors)."""
+    dtc = _dtc()
+    for doms in ([_binary(dtc, 24, 121), _binary(dtc, 28, 122)], [_grey(dtc, 20, 123)]):
+        kw = dict(precision=1, rel_tol=1e-8, rotation=False, precond=dtc.DTC_PC_MG)
+        monkeypatch.setenv("DTC_ORTH_FUSE", "0")
+        ra = dtc.solve_batch(doms, **kw)
+        monkeypatch.delenv("DTC_ORTH_FUSE")
+        rb = dtc.solve_batch(doms, **kw)
+        for a, c in zip(ra, rb):
+            assert a["status"] == c["status"] == 0
+            assert abs(a["iterations"] - c["iterations"]) <= 2
+            for k in ("C", "K_macro", "ff"):
+                assert np.abs(a[k] - c[k]).max() <= 1e-7 * np.abs(a[k]).max(), k
